@@ -1,0 +1,198 @@
+/*
+ * raygauss_hip.h -- C ABI of the MI355X (gfx950) differentiable Gaussian ray caster.
+ *
+ * This library replaces, for the RayGauss training path, the OptiX/CuPy machinery behind
+ * RenderOptixFunction (reference optix_raycasting/render_optix.py:90-190).  Every entry point
+ * below names the reference interface it stands in for.  The ABI uses plain pointers and sizes
+ * only: all array pointers are DEVICE pointers (hipMalloc'd or torch-allocated), cameras are
+ * host structs, `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Error handling: every function returns 0 on success and a negative rg_status otherwise; the
+ * message is available from rg_last_error(handle).  Nothing throws across the ABI.
+ * Threading: a handle is used from one host thread at a time (one handle per stream).
+ * Ownership: the caller owns inputs and outputs; the handle owns its workspace (per-Gaussian
+ * records, LBVH, per-view colour tables, gradient accumulators), grown on demand.
+ */
+#ifndef RAYGAUSS_HIP_H
+#define RAYGAUSS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rg_status {
+    RG_OK = 0,
+    RG_ERR_ARG = -1,       /* bad argument (null pointer, size, unsupported degree ...) */
+    RG_ERR_HIP = -2,       /* HIP runtime error */
+    RG_ERR_OOM = -3,       /* device allocation failed */
+    RG_ERR_STATE = -4,     /* call order (e.g. forward before prepare) */
+    RG_ERR_OVERFLOW = -5   /* max_hits_per_slab > 0 and a slab exceeded it */
+} rg_status;
+
+typedef struct rg_handle rg_handle;
+
+/* Runtime form of the reference's compile-time constants
+ * (optix_raycasting/cuda_train/forward/gaussians_aabb.h:2-5) plus launch tuning. */
+typedef struct rg_config {
+    float dt;                  /* sample spacing                 (reference DT = 0.0025)       */
+    int32_t samples_per_slab;  /* samples per slab, 1..16         (reference BUFFER_SIZE = 8)   */
+    float t_eps;               /* transmittance cut-off          (TRANSMITTANCE_EPSILON 0.003) */
+    float sigma_thr;           /* density threshold              (SIGMA_THRESHOLD = 0.1)       */
+    float t_near;              /* ray interval clip; 0 = reference                               */
+    float t_far;               /* ray interval clip; +inf = reference                            */
+    int32_t white_background;  /* RenderOptixSettings.white_background (render_optix.py:70)     */
+    int32_t exact_sh_grad;     /* 0: backward_sh as the reference (utils/sh_utils.cu:254)        */
+    int32_t hit_depth;         /* per-ray interval list capacity: 8, 16 or 32                    */
+    int32_t seg_slabs;         /* initial slabs per interval gather: 1..4096                     */
+    int32_t chunk;             /* rays per work item in the persistent kernels: 64, 128, 256    */
+    int32_t max_hits_per_slab; /* >0: count slabs whose hit set exceeds it (max_prim_slice)      */
+} rg_config;
+
+/* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of
+ * world_view_transform[:3,:3]; tanx/tany = tan(FoV/2). */
+typedef struct rg_camera {
+    float eye[3];
+    float u[3];
+    float v[3];
+    float w[3];
+    float tanx;
+    float tany;
+    int32_t width;
+    int32_t height;
+} rg_camera;
+
+/* The boundary inputs of RenderOptixFunction.forward (render_optix.py:92-93), already activated
+ * (classes/point_cloud.py:729-739).  Layouts: positions (n,3), scales (n,3), quats (n,4) w,x,y,z,
+ * densities (n), sh (n,3,K) K=(deg+1)^2, sg (n,3,L), sharpness (n,L), lobe_axis (n,L,3). */
+typedef struct rg_gaussians {
+    int64_t n;
+    int32_t sh_degree;  /* 0..3 */
+    int32_t num_sg;     /* L >= 0; sg/sharpness/lobe_axis may be NULL when 0 */
+    const float* positions;
+    const float* scales;
+    const float* quats;
+    const float* densities;
+    const float* sh;
+    const float* sg;
+    const float* sharpness;
+    const float* lobe_axis;
+} rg_gaussians;
+
+/* Gradient outputs, original Gaussian order, same layouts as rg_gaussians (overwritten).
+ * The sg/sharpness/lobe_axis pointers may be NULL when num_sg == 0. */
+typedef struct rg_grads {
+    float* positions;
+    float* scales;
+    float* quats;
+    float* densities;
+    float* sh;
+    float* sg;
+    float* sharpness;
+    float* lobe_axis;
+} rg_grads;
+
+/* A batch of rays over the views given to rg_set_views: ray r is pixel ray_pixel[r] of view
+ * ray_view[r] (device int32 arrays).  This is the multi-view generalisation of the reference's
+ * one-camera launch (optix_utils.py:132,201). */
+typedef struct rg_ray_batch {
+    int64_t num_rays;
+    const int32_t* ray_view;
+    const int32_t* ray_pixel;
+} rg_ray_batch;
+
+/* Per-call counters (device-side, read back by rg_get_stats). */
+typedef struct rg_stats {
+    int64_t slab_hits;        /* K: sum over processed slabs of |hit set|                     */
+    int64_t pairs;            /* P: distinct (ray, Gaussian) pairs                            */
+    int64_t overflow_slabs;   /* slabs with more hits than max_hits_per_slab (optix_utils:205) */
+    int64_t fallback_slabs;   /* slabs whose hit set exceeded hit_depth (per-slab traversal)   */
+    int64_t gathers;          /* interval-list gathers (BVH traversals)                       */
+} rg_stats;
+
+/* Creates a handle.  Replaces create_context/module/pipeline (optix_utils.py:77-110): the
+ * kernels are compiled ahead of time for gfx950. */
+int rg_create(rg_handle** out, const rg_config* cfg);
+int rg_destroy(rg_handle* h);
+int rg_set_config(rg_handle* h, const rg_config* cfg);
+const char* rg_last_error(const rg_handle* h);
+
+/* Per-Gaussian preprocessing and acceleration structure.  Replaces quaternion_to_rotation +
+ * compute_ellipsoids_bbox + the scene-box reductions + create/update_acceleration_structure
+ * (optix_utils.py:34-75, render_optix.py:103-110).  rebuild != 0 (or a change of n) rebuilds
+ * the LBVH (Morton codes, radix sort, radix tree); otherwise the existing tree is refit. */
+int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* stream);
+
+/* Uploads the views and computes each view's per-Gaussian colour table.  Replaces
+ * compute_cupy_rgb / forward_sh (render_optix.py:19-32, utils/sh_utils.cu:25-119) and the
+ * camera part of the launch record (optix_utils.py:166-181). */
+int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* stream);
+
+/* Forward render of num_rays rays of one view (pixels == NULL: ray r is pixel r).  out_rgb
+ * (num_rays,3) is required; out_depth / out_opacity (num_rays) are optional.  Replaces
+ * launch_pipeline_forward (optix_utils.py:128-209) and the forward raygen
+ * (cuda_train/forward/gaussians_aabb.cu:521-612). */
+int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, float* out_rgb,
+               float* out_depth, float* out_opacity, void* stream);
+
+/* Backward of rg_forward for the same rays: accumulates parameter gradients into the handle.
+ * ray_colors are rg_forward's outputs, dL_dC the incoming gradient (num_rays,3).  Replaces
+ * launch_pipeline_backward (optix_utils.py:213-299) and the backward raygen
+ * (cuda_train/backward/gaussians_aabb.cu:505-775). */
+int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, const float* ray_colors,
+                const float* dL_dC, void* stream);
+
+/* Multi-view versions (persistent kernels pulling ray chunks from a device work queue). */
+int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float* out_depth, float* out_opacity,
+                     void* stream);
+int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colors, const float* dL_dC,
+                      void* stream);
+
+/* Fused train step over a batch: forward, per-ray loss against target (loss_kind 0: L2 =
+ * sum (C-target)^2, 1: L1 = sum |C-target|), backward -- in one persistent kernel, with no
+ * intermediate colour buffer.  out_rgb (optional) receives the rendered colours, out_loss
+ * (optional, device float[1]) the summed loss.  No reference counterpart: it fuses
+ * RenderOptixFunction.forward + the caller's per-pixel loss + backward (scripts/train.py:104-114)
+ * for per-pixel losses. */
+int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int32_t loss_kind, float* out_rgb,
+                   float* out_loss, void* stream);
+
+/* Zeroes the gradient accumulators (done automatically by rg_finish_grads). */
+int rg_zero_grads(rg_handle* h, void* stream);
+
+/* Chains the accumulated per-view colour gradients through the SH/SG model (backward_sh,
+ * utils/sh_utils.cu:121-315, summed over views), adds the view-direction position term
+ * (render_optix.py:175), writes all gradients in the original Gaussian order and clears the
+ * accumulators.  Replaces compute_cupy_sh_grad + the gradient assembly (render_optix.py:34-52,
+ * :170-190). */
+int rg_finish_grads(rg_handle* h, const rg_grads* out, void* stream);
+
+/* Device scene box (6 floats: min xyz, max xyz) of the last rg_prepare: the reference's
+ * bb_min/bb_max (render_optix.py:105-106). */
+int rg_scene_box(rg_handle* h, float* out6_device, void* stream);
+
+/* Counters of the last forward/backward/train call (synchronises the stream). */
+int rg_get_stats(rg_handle* h, rg_stats* out, void* stream);
+
+/* Debug/verification: the FP32 predicate parameters of the bounding ellipsoids in original
+ * order -- mu (n,3), Mp (n,9) rows of S'^-1 R^T, valid (n) -- exactly as the kernels use them. */
+int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, void* stream);
+
+/* Debug/verification: per-slab hit sets found by the LBVH traversal for num_rays rays of a view,
+ * enumerating slabs from tenter to texit without transmittance exit.  pairs (num_rays,
+ * max_pairs, 2) receives (slab index, original Gaussian index) sorted, counts (num_rays) the
+ * number of pairs (may exceed max_pairs).  Also returns the FP32 rays and their interval. */
+int rg_debug_hits(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, int32_t max_slabs,
+                  int32_t max_pairs, int32_t* pairs, int64_t* counts, float* rays_o, float* rays_d,
+                  float* tenter, float* texit, void* stream);
+
+/* LBVH structural check (debug): returns 0 when every node box contains its children's boxes
+ * and every leaf appears exactly once; nonzero counts errors. */
+int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAYGAUSS_HIP_H */

@@ -1,0 +1,97 @@
+"""ctypes binding of the C ABI in include/raygauss_hip.h (libraygauss_hip.so).
+
+The product path has no CPU fallback: if the library is missing this module raises, and every
+entry point reports HIP errors as exceptions.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libraygauss_hip.so")
+
+_lib = None
+
+
+class Config(C.Structure):
+    _fields_ = [("dt", C.c_float), ("samples_per_slab", C.c_int32), ("t_eps", C.c_float), ("sigma_thr", C.c_float),
+                ("t_near", C.c_float), ("t_far", C.c_float), ("white_background", C.c_int32),
+                ("exact_sh_grad", C.c_int32), ("hit_depth", C.c_int32), ("seg_slabs", C.c_int32),
+                ("chunk", C.c_int32), ("max_hits_per_slab", C.c_int32)]
+
+
+class CameraS(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("w", C.c_float * 3),
+                ("tanx", C.c_float), ("tany", C.c_float), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class Gaussians(C.Structure):
+    _fields_ = [("n", C.c_int64), ("sh_degree", C.c_int32), ("num_sg", C.c_int32)] + [
+        (k, C.c_void_p) for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis")]
+
+
+class Grads(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness",
+                                          "lobe_axis")]
+
+
+class RayBatch(C.Structure):
+    _fields_ = [("num_rays", C.c_int64), ("ray_view", C.c_void_p), ("ray_pixel", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("slab_hits", C.c_int64), ("pairs", C.c_int64), ("overflow_slabs", C.c_int64),
+                ("fallback_slabs", C.c_int64), ("gathers", C.c_int64)]
+
+
+EXPORTS = {
+    "rg_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Config)]),
+    "rg_destroy": (C.c_int, [C.c_void_p]),
+    "rg_set_config": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+    "rg_last_error": (C.c_char_p, [C.c_void_p]),
+    "rg_prepare": (C.c_int, [C.c_void_p, C.POINTER(Gaussians), C.c_int32, C.c_void_p]),
+    "rg_set_views": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CameraS), C.c_void_p]),
+    "rg_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p]),
+    "rg_backward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_forward_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_backward_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_train_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
+    "rg_zero_grads": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "rg_finish_grads": (C.c_int, [C.c_void_p, C.POINTER(Grads), C.c_void_p]),
+    "rg_scene_box": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats), C.c_void_p]),
+    "rg_export_predicate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_debug_hits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_debug_check_bvh": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+}
+
+
+def load():
+    """Loads libraygauss_hip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class RayGaussError(RuntimeError):
+    pass
+
+
+def check(rc: int, handle) -> None:
+    if rc != 0:
+        msg = load().rg_last_error(handle)
+        raise RayGaussError(f"raygauss_hip error {rc}: {msg.decode() if msg else ''}")

@@ -1,0 +1,470 @@
+// C ABI of the gfx950 ray caster (declared in include/raygauss_hip.h).  Owns the per-handle
+// workspace and sequences the kernels; never throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rg_internal.h"
+
+struct rg_handle {
+    rg_config cfg;
+    std::string err;
+    // Gaussians
+    int64_t n = 0;
+    int64_t cap = 0;
+    bool built = false;
+    rg_gaussians g{};
+    int generation = 0;
+    // per-Gaussian buffers (slot order)
+    float4* geo = nullptr;
+    float4* brec = nullptr;
+    float4* leafbox = nullptr;
+    float* acc = nullptr;
+    float* box = nullptr;
+    unsigned long long* counters = nullptr;
+    rg::BvhBuffers bvh{};
+    int64_t hist_cap = 0;
+    // views
+    int num_views = 0;
+    int views_cap = 0;
+    rg_camera* cams = nullptr;
+    std::vector<rg_camera> host_cams;
+    float4* rgb = nullptr;
+    float4* drgb = nullptr;
+    int64_t rgb_cap = 0;  // floats4 allocated for rgb and drgb
+};
+
+namespace {
+
+int fail(rg_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(h, RG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+template <typename T>
+int dalloc(rg_handle* h, T** p, int64_t count) {
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(h, RG_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return RG_OK;
+}
+
+int check_config(rg_handle* h, const rg_config* c) {
+    if (!c) return fail(h, RG_ERR_ARG, "null config");
+    if (!(c->dt > 0.0f) || !std::isfinite(c->dt)) return fail(h, RG_ERR_ARG, "dt must be positive");
+    if (c->samples_per_slab < 1 || c->samples_per_slab > 16) return fail(h, RG_ERR_ARG, "samples_per_slab in 1..16");
+    if (!(c->sigma_thr > 0.0f)) return fail(h, RG_ERR_ARG, "sigma_thr must be positive");
+    if (!(c->t_near <= c->t_far)) return fail(h, RG_ERR_ARG, "t_near must be <= t_far");
+    if (c->hit_depth != 8 && c->hit_depth != 16 && c->hit_depth != 32)
+        return fail(h, RG_ERR_ARG, "hit_depth must be 8, 16 or 32");
+    if (c->seg_slabs < 1 || c->seg_slabs > 4096) return fail(h, RG_ERR_ARG, "seg_slabs in 1..4096");
+    if (c->chunk != 64 && c->chunk != 128 && c->chunk != 256) return fail(h, RG_ERR_ARG, "chunk must be 64, 128 or 256");
+    return RG_OK;
+}
+
+int ensure_gaussian_capacity(rg_handle* h, int64_t n) {
+    if (n <= h->cap && h->geo) return RG_OK;
+    const int64_t cap = n;
+    const int64_t total = 2 * cap;  // >= 2n-1
+    int rc;
+    if ((rc = dalloc(h, &h->geo, 4 * cap))) return rc;
+    if ((rc = dalloc(h, &h->brec, 2 * cap))) return rc;
+    if ((rc = dalloc(h, &h->leafbox, 2 * cap))) return rc;
+    if ((rc = dalloc(h, &h->acc, rg::kAccStride * cap))) return rc;
+    rg::BvhBuffers& b = h->bvh;
+    if ((rc = dalloc(h, &b.keys[0], cap))) return rc;
+    if ((rc = dalloc(h, &b.keys[1], cap))) return rc;
+    if ((rc = dalloc(h, &b.vals[0], cap))) return rc;
+    if ((rc = dalloc(h, &b.vals[1], cap))) return rc;
+    const int64_t tiles = (cap + 2047) / 2048;
+    if ((rc = dalloc(h, &b.hist, 256 * tiles))) return rc;
+    if ((rc = dalloc(h, &b.perm, cap))) return rc;
+    if ((rc = dalloc(h, &b.iperm, cap))) return rc;
+    if ((rc = dalloc(h, &b.nodes, 2 * total))) return rc;
+    if ((rc = dalloc(h, &b.parent, total))) return rc;
+    if ((rc = dalloc(h, &b.right, total))) return rc;
+    if ((rc = dalloc(h, &b.counter, total))) return rc;
+    if ((rc = dalloc(h, &b.leaf_node, cap))) return rc;
+    if ((rc = dalloc(h, &b.k_parent, total))) return rc;
+    if ((rc = dalloc(h, &b.k_left, cap))) return rc;
+    if ((rc = dalloc(h, &b.k_right, cap))) return rc;
+    if ((rc = dalloc(h, &b.k_first, cap))) return rc;
+    if ((rc = dalloc(h, &b.k_last, cap))) return rc;
+    h->cap = cap;
+    h->built = false;
+    return RG_OK;
+}
+
+int ensure_view_capacity(rg_handle* h, int views) {
+    const int64_t need = (int64_t)views * std::max<int64_t>(h->n, 1);
+    int rc;
+    if (views > h->views_cap || !h->cams) {
+        if ((rc = dalloc(h, &h->cams, views))) return rc;
+        h->views_cap = views;
+    }
+    if (need > h->rgb_cap || !h->rgb) {
+        if ((rc = dalloc(h, &h->rgb, need))) return rc;
+        if ((rc = dalloc(h, &h->drgb, need))) return rc;
+        h->rgb_cap = need;
+        HIPCHK(h, hipMemset(h->drgb, 0, sizeof(float4) * need));
+    }
+    return RG_OK;
+}
+
+rg::DevScene dev_scene(rg_handle* h) {
+    rg::DevScene s;
+    s.n = (int)h->n;
+    s.total_nodes = h->n > 0 ? (int)(2 * h->n - 1) : 0;
+    s.geo = h->geo;
+    s.brec = h->brec;
+    s.nodes = h->bvh.nodes;
+    s.rgb = h->rgb;
+    s.box = h->box;
+    s.perm = h->bvh.perm;
+    return s;
+}
+
+rg::DevParams dev_params(const rg_handle* h) {
+    rg::DevParams p;
+    p.dt = h->cfg.dt;
+    p.spb = h->cfg.samples_per_slab;
+    p.t_eps = h->cfg.t_eps;
+    p.sigma_thr = h->cfg.sigma_thr;
+    p.t_near = h->cfg.t_near;
+    p.t_far = h->cfg.t_far;
+    p.bg = h->cfg.white_background ? 1.0f : 0.0f;
+    p.max_hits = h->cfg.max_hits_per_slab;
+    p.seg_slabs = h->cfg.seg_slabs;
+    return p;
+}
+
+int ready_for_rays(rg_handle* h, int view) {
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    if (h->num_views <= 0) return fail(h, RG_ERR_STATE, "rg_set_views has not been called");
+    if (view < 0 || view >= h->num_views) return fail(h, RG_ERR_ARG, "view index out of range");
+    return RG_OK;
+}
+
+rg::RayJob single_job(rg_handle* h, int view, int64_t num_rays, const int32_t* pixels) {
+    rg::RayJob j;
+    j.pixels = pixels;
+    j.ray_view = nullptr;
+    j.ray_pixel = nullptr;
+    j.view = view;
+    j.cams = h->cams;
+    j.num_views = h->num_views;
+    if (pixels) {
+        j.num_rays = num_rays;
+    } else {
+        // implicit full view: 8x8 tiles
+        const rg_camera& c = h->host_cams[view];
+        j.num_rays = (int64_t)((c.width + 7) / 8) * ((c.height + 7) / 8) * 64;
+    }
+    return j;
+}
+
+rg::DevStats dev_stats(rg_handle* h) {
+    rg::DevStats s;
+    s.c = h->counters;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rg_create(rg_handle** out, const rg_config* cfg) {
+    if (!out) return RG_ERR_ARG;
+    *out = nullptr;
+    rg_handle* h = new (std::nothrow) rg_handle();
+    if (!h) return RG_ERR_OOM;
+    int rc = check_config(h, cfg);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    h->cfg = *cfg;
+    if ((rc = dalloc(h, &h->box, 8)) || (rc = dalloc(h, &h->counters, 8))) {
+        delete h;
+        return rc;
+    }
+    if (hipMemset(h->counters, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        delete h;
+        return RG_ERR_HIP;
+    }
+    *out = h;
+    return RG_OK;
+}
+
+int rg_destroy(rg_handle* h) {
+    if (!h) return RG_OK;
+    void* ptrs[] = {h->geo, h->brec, h->leafbox, h->acc, h->box, h->counters, h->bvh.keys[0], h->bvh.keys[1],
+                    h->bvh.vals[0], h->bvh.vals[1], h->bvh.hist, h->bvh.perm, h->bvh.iperm, h->bvh.nodes,
+                    h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
+                    h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete h;
+    return RG_OK;
+}
+
+int rg_set_config(rg_handle* h, const rg_config* cfg) {
+    if (!h) return RG_ERR_ARG;
+    int rc = check_config(h, cfg);
+    if (rc) return rc;
+    const bool thr_changed = cfg->sigma_thr != h->cfg.sigma_thr;
+    h->cfg = *cfg;
+    if (thr_changed) h->built = false;  // records depend on the threshold: prepare again
+    return RG_OK;
+}
+
+const char* rg_last_error(const rg_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* stream) {
+    if (!h || !g) return fail(h, RG_ERR_ARG, "null argument");
+    if (g->n < 0 || g->n > (int64_t)0x3fffffff) return fail(h, RG_ERR_ARG, "n out of range");
+    if (g->sh_degree < 0 || g->sh_degree > 3) return fail(h, RG_ERR_ARG, "sh_degree must be 0..3");
+    if (g->num_sg < 0 || g->num_sg > 8) return fail(h, RG_ERR_ARG, "num_sg must be 0..8");
+    if (g->n > 0 && (!g->positions || !g->scales || !g->quats || !g->densities || !g->sh))
+        return fail(h, RG_ERR_ARG, "null Gaussian array");
+    if (g->n > 0 && g->num_sg > 0 && (!g->sg || !g->sharpness || !g->lobe_axis))
+        return fail(h, RG_ERR_ARG, "null SG array with num_sg > 0");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    if ((rc = ensure_gaussian_capacity(h, g->n))) return rc;
+    const bool topo = rebuild || !h->built || g->n != h->n;
+    h->n = g->n;
+    h->g = *g;
+    h->generation++;
+    const int n = (int)g->n;
+    if (topo) {
+        // records in original order -> Morton order -> tree
+        HIPCHK(h, rg::launch_iota(n, h->bvh.perm, h->bvh.iperm, s));
+        HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
+        HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
+        HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
+    }
+    HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
+    HIPCHK(h, rg::launch_lbvh_refit(n, h->leafbox, h->bvh, s));
+    h->built = true;
+    if (topo) h->num_views = 0;  // colour tables are per slot: views must be set again
+    else if (h->num_views > 0) {
+        HIPCHK(h, rg::launch_sh_forward(h->g, h->bvh.iperm, h->cams, h->num_views, h->rgb, s));
+    }
+    return RG_OK;
+}
+
+int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* stream) {
+    if (!h || !cams) return fail(h, RG_ERR_ARG, "null argument");
+    if (num_views <= 0) return fail(h, RG_ERR_ARG, "num_views must be positive");
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    for (int v = 0; v < num_views; ++v)
+        if (cams[v].width <= 0 || cams[v].height <= 0) return fail(h, RG_ERR_ARG, "camera size must be positive");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    if ((rc = ensure_view_capacity(h, num_views))) return rc;
+    h->host_cams.assign(cams, cams + num_views);
+    HIPCHK(h, hipMemcpyAsync(h->cams, cams, sizeof(rg_camera) * num_views, hipMemcpyHostToDevice, s));
+    h->num_views = num_views;
+    HIPCHK(h, rg::launch_sh_forward(h->g, h->bvh.iperm, h->cams, num_views, h->rgb, s));
+    return RG_OK;
+}
+
+int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, float* out_rgb, float* out_depth,
+               float* out_opacity, void* stream) {
+    if (!h || !out_rgb) return fail(h, RG_ERR_ARG, "null argument");
+    int rc;
+    if ((rc = ready_for_rays(h, view))) return rc;
+    if (num_rays < 0) return fail(h, RG_ERR_ARG, "num_rays < 0");
+    const rg_camera& c = h->host_cams[view];
+    if (!pixels && num_rays != (int64_t)c.width * c.height)
+        return fail(h, RG_ERR_ARG, "without a pixel list num_rays must equal width*height");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
+    const rg::RayJob job = single_job(h, view, num_rays, pixels);
+    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, out_rgb, out_depth, out_opacity,
+                                 dev_stats(h), s));
+    return RG_OK;
+}
+
+int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, const float* ray_colors,
+                const float* dL_dC, void* stream) {
+    if (!h || !ray_colors || !dL_dC) return fail(h, RG_ERR_ARG, "null argument");
+    int rc;
+    if ((rc = ready_for_rays(h, view))) return rc;
+    const rg_camera& c = h->host_cams[view];
+    if (!pixels && num_rays != (int64_t)c.width * c.height)
+        return fail(h, RG_ERR_ARG, "without a pixel list num_rays must equal width*height");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
+    const rg::RayJob job = single_job(h, view, num_rays, pixels);
+    rg::DevGrad gr{h->acc, h->drgb};
+    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, ray_colors, dL_dC, gr,
+                                  dev_stats(h), s));
+    return RG_OK;
+}
+
+static int batch_job(rg_handle* h, const rg_ray_batch* b, rg::RayJob& j) {
+    if (!b) return fail(h, RG_ERR_ARG, "null batch");
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    if (h->num_views <= 0) return fail(h, RG_ERR_STATE, "rg_set_views has not been called");
+    if (b->num_rays < 0 || (b->num_rays > 0 && (!b->ray_view || !b->ray_pixel)))
+        return fail(h, RG_ERR_ARG, "bad ray batch");
+    j.num_rays = b->num_rays;
+    j.pixels = nullptr;
+    j.ray_view = b->ray_view;
+    j.ray_pixel = b->ray_pixel;
+    j.view = 0;
+    j.cams = h->cams;
+    j.num_views = h->num_views;
+    return RG_OK;
+}
+
+int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float* out_depth, float* out_opacity,
+                     void* stream) {
+    if (!h || !out_rgb) return fail(h, RG_ERR_ARG, "null argument");
+    rg::RayJob j;
+    int rc;
+    if ((rc = batch_job(h, b, j))) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
+    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, out_rgb, out_depth, out_opacity,
+                                 dev_stats(h), s));
+    return RG_OK;
+}
+
+int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colors, const float* dL_dC, void* stream) {
+    if (!h || !ray_colors || !dL_dC) return fail(h, RG_ERR_ARG, "null argument");
+    rg::RayJob j;
+    int rc;
+    if ((rc = batch_job(h, b, j))) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
+    rg::DevGrad gr{h->acc, h->drgb};
+    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, ray_colors, dL_dC, gr,
+                                  dev_stats(h), s));
+    return RG_OK;
+}
+
+int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int32_t loss_kind, float* out_rgb,
+                   float* out_loss, void* stream) {
+    if (!h || !target) return fail(h, RG_ERR_ARG, "null argument");
+    if (loss_kind != 0 && loss_kind != 1) return fail(h, RG_ERR_ARG, "loss_kind must be 0 (L2) or 1 (L1)");
+    rg::RayJob j;
+    int rc;
+    if ((rc = batch_job(h, b, j))) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
+    rg::DevGrad gr{h->acc, h->drgb};
+    HIPCHK(h, rg::launch_train(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, h->cfg.chunk, target, loss_kind,
+                               out_rgb, out_loss, gr, dev_stats(h), s));
+    if (out_loss)
+        HIPCHK(h, hipMemcpyAsync(out_loss, h->counters + 6, sizeof(float), hipMemcpyDeviceToDevice, s));
+    return RG_OK;
+}
+
+int rg_zero_grads(rg_handle* h, void* stream) {
+    if (!h) return RG_ERR_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->acc) HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * h->cap, s));
+    if (h->drgb) HIPCHK(h, rg::launch_zero(reinterpret_cast<float*>(h->drgb), 4 * h->rgb_cap, s));
+    return RG_OK;
+}
+
+int rg_finish_grads(rg_handle* h, const rg_grads* out, void* stream) {
+    if (!h || !out) return fail(h, RG_ERR_ARG, "null argument");
+    if (!h->built || h->num_views <= 0) return fail(h, RG_ERR_STATE, "nothing to finish");
+    if (h->n > 0 && (!out->positions || !out->scales || !out->quats || !out->densities || !out->sh))
+        return fail(h, RG_ERR_ARG, "null gradient array");
+    if (h->n > 0 && h->g.num_sg > 0 && (!out->sg || !out->sharpness || !out->lobe_axis))
+        return fail(h, RG_ERR_ARG, "null SG gradient array with num_sg > 0");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, rg::launch_finish_grads(h->g, h->bvh.iperm, h->cams, h->num_views, h->acc, h->drgb,
+                                      h->cfg.exact_sh_grad, *out, s));
+    return RG_OK;
+}
+
+int rg_scene_box(rg_handle* h, float* out6, void* stream) {
+    if (!h || !out6) return fail(h, RG_ERR_ARG, "null argument");
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    HIPCHK(h, hipMemcpyAsync(out6, h->box, 6 * sizeof(float), hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+    return RG_OK;
+}
+
+int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
+    if (!h || !out) return fail(h, RG_ERR_ARG, "null argument");
+    unsigned long long c[8];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPCHK(h, hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    out->slab_hits = (int64_t)c[0];
+    out->pairs = (int64_t)c[1];
+    out->overflow_slabs = (int64_t)c[2];
+    out->fallback_slabs = (int64_t)c[3];
+    out->gathers = (int64_t)c[4];
+    return RG_OK;
+}
+
+int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, void* stream) {
+    if (!h || !mu || !Mp || !valid) return fail(h, RG_ERR_ARG, "null argument");
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    HIPCHK(h, rg::launch_export_predicate((int)h->n, h->bvh.iperm, h->geo, mu, Mp, valid,
+                                          static_cast<hipStream_t>(stream)));
+    return RG_OK;
+}
+
+int rg_debug_hits(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, int32_t max_slabs,
+                  int32_t max_pairs, int32_t* pairs, int64_t* counts, float* rays_o, float* rays_d, float* tenter,
+                  float* texit, void* stream) {
+    if (!h || !pairs || !counts || !rays_o || !rays_d || !tenter || !texit) return fail(h, RG_ERR_ARG, "null argument");
+    int rc;
+    if ((rc = ready_for_rays(h, view))) return rc;
+    if (max_pairs <= 0 || max_slabs <= 0) return fail(h, RG_ERR_ARG, "max_pairs and max_slabs must be positive");
+    const rg_camera& c = h->host_cams[view];
+    if (!pixels && num_rays > (int64_t)c.width * c.height) return fail(h, RG_ERR_ARG, "num_rays > width*height");
+    rg::RayJob j;
+    j.num_rays = num_rays;
+    j.pixels = pixels;
+    j.ray_view = nullptr;
+    j.ray_pixel = nullptr;
+    j.view = view;
+    j.cams = h->cams;
+    j.num_views = h->num_views;
+    HIPCHK(h, rg::launch_debug_hits(dev_scene(h), dev_params(h), j, max_slabs, max_pairs, pairs, counts, rays_o, rays_d,
+                                    tenter, texit, static_cast<hipStream_t>(stream)));
+    return RG_OK;
+}
+
+int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream) {
+    if (!h || !errors) return fail(h, RG_ERR_ARG, "null argument");
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long* d = h->counters + 7;
+    HIPCHK(h, hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+    HIPCHK(h, rg::launch_lbvh_check((int)h->n, h->bvh, d, s));
+    unsigned long long e = 0;
+    HIPCHK(h, hipMemcpyAsync(&e, d, sizeof(e), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    *errors = (int64_t)e;
+    return RG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// Device-side building blocks of the ray caster: FP32 ray setup, the bounding-ellipsoid
+// predicate, box tests and the stackless (threaded) LBVH traversal.
+//
+// Functions marked "exact FP32" are evaluated with contraction off so that every operation
+// rounds once; they define the slab grid and the hit sets, which must be reproducible bit for
+// bit by an FP32 restatement on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rg_internal.h"
+
+namespace rg {
+
+struct Ray {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float tenter, texit;
+};
+
+// computeRay (reference cuda_train/forward/gaussians_aabb.cu:405-434, jitter off) followed by
+// the scene-box slab test (:530-541) clipped to [t_near, t_far].  Exact FP32.
+__device__ __forceinline__ Ray make_ray(const rg_camera& c, int pixel, const float* box, float t_near, float t_far) {
+#pragma clang fp contract(off)
+    Ray r;
+    const int ix = pixel % c.width;
+    const int iy = pixel / c.width;
+    const float fx = ((float)ix + 0.5f) / (float)c.width;
+    const float fy = ((float)iy + 0.5f) / (float)c.height;
+    const float px = 2.0f * fx - 1.0f;
+    const float py = 2.0f * fy - 1.0f;
+    const float ddx = px * c.tanx;
+    const float ddy = py * c.tany;
+    float v0 = (ddx * c.u[0] + ddy * c.v[0]) + c.w[0];
+    float v1 = (ddx * c.u[1] + ddy * c.v[1]) + c.w[1];
+    float v2 = (ddx * c.u[2] + ddy * c.v[2]) + c.w[2];
+    const float len2 = (v0 * v0 + v1 * v1) + v2 * v2;
+    const float inv = 1.0f / __builtin_sqrtf(len2);
+    r.ox = c.eye[0];
+    r.oy = c.eye[1];
+    r.oz = c.eye[2];
+    r.dx = v0 * inv;
+    r.dy = v1 * inv;
+    r.dz = v2 * inv;
+    const float t0x = (box[0] - r.ox) / r.dx, t1x = (box[3] - r.ox) / r.dx;
+    const float t0y = (box[1] - r.oy) / r.dy, t1y = (box[4] - r.oy) / r.dy;
+    const float t0z = (box[2] - r.oz) / r.dz, t1z = (box[5] - r.oz) / r.dz;
+    const float mnx = fminf(t0x, t1x), mxx = fmaxf(t0x, t1x);
+    const float mny = fminf(t0y, t1y), mxy = fmaxf(t0y, t1y);
+    const float mnz = fminf(t0z, t1z), mxz = fmaxf(t0z, t1z);
+    r.tenter = fmaxf(t_near, fmaxf(mnx, fmaxf(mny, mnz)));
+    r.texit = fminf(t_far, fminf(mxx, fminf(mxy, mxz)));
+    return r;
+}
+
+// Exact FP32: next slab start (reference accumulates tbuffer += slab_spacing, :596).
+__device__ __forceinline__ float add_exact(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// Exact FP32: t of sample i in the slab starting at tb (reference :474).
+__device__ __forceinline__ float sample_t(float tb, int i, float dt) {
+#pragma clang fp contract(off)
+    const float off = (float)i * dt;
+    return tb + off;
+}
+
+// Per (ray, Gaussian) result of the bounding-ellipsoid test.
+struct Isect {
+    float r1, r2;  // predicate interval (root1, root2)
+    float tc;      // t of closest approach in ellipsoid metric (b * l)
+    float A;       // h * |P|^2          -> weight exponent p(t) = A + Cq (t - tc)^2
+    float Cq;      // h * |D'|^2
+    float Px, Py, Pz;     // O' + b D^   (ellipsoid-space offset at closest approach)
+    float Dx, Dy, Dz;     // D' = M' d
+};
+
+// __intersection__gaussian (reference cuda_train/forward/gaussians_aabb.cu:324-402), exact FP32
+// for the decision part.  Returns true when disc > 1e-7; the hit test on [tmin, tmax] is then
+// fmaxf(tmin, r1) <= fminf(tmax, r2).  The operation order is shared with the host FP32 checker.
+__device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
+                                          const Ray& r, Isect& I) {
+#pragma clang fp contract(off)
+    if (!(g1.w > 0.0f)) return false;  // h <= 0 or NaN: no bounding ellipsoid (|sigma| <= thr)
+    const float Ox = r.ox - g0.x, Oy = r.oy - g0.y, Oz = r.oz - g0.z;
+    const float Op0 = (g1.x * Ox + g1.y * Oy) + g1.z * Oz;
+    const float Op1 = (g2.x * Ox + g2.y * Oy) + g2.z * Oz;
+    const float Op2 = (g3.x * Ox + g3.y * Oy) + g3.z * Oz;
+    const float Dp0 = (g1.x * r.dx + g1.y * r.dy) + g1.z * r.dz;
+    const float Dp1 = (g2.x * r.dx + g2.y * r.dy) + g2.z * r.dz;
+    const float Dp2 = (g3.x * r.dx + g3.y * r.dy) + g3.z * r.dz;
+    const float ddp = (Dp0 * Dp0 + Dp1 * Dp1) + Dp2 * Dp2;
+    const float l = 1.0f / __builtin_sqrtf(ddp);
+    const float Dh0 = Dp0 * l, Dh1 = Dp1 * l, Dh2 = Dp2 * l;
+    const float b = -((Op0 * Dh0 + Op1 * Dh1) + Op2 * Dh2);
+    const float c = ((Op0 * Op0 + Op1 * Op1) + Op2 * Op2) - 1.0f;
+    const float P0 = Op0 + b * Dh0, P1 = Op1 + b * Dh1, P2 = Op2 + b * Dh2;
+    const float pp = (P0 * P0 + P1 * P1) + P2 * P2;
+    const float disc = 1.0f - pp;
+    if (!(disc > 1e-7f)) return false;
+    const float sd = __builtin_sqrtf(disc);
+    const float q = (b > 0.0f) ? b + sd : b - sd;
+    I.r1 = (c / q) * l;
+    I.r2 = q * l;
+    I.tc = b * l;
+    I.A = g1.w * pp;
+    I.Cq = g1.w * ddp;
+    I.Px = P0;
+    I.Py = P1;
+    I.Pz = P2;
+    I.Dx = Dp0;
+    I.Dy = Dp1;
+    I.Dz = Dp2;
+    return true;
+}
+
+__device__ __forceinline__ bool overlaps(float tmin, float tmax, float r1, float r2) {
+    return fmaxf(tmin, r1) <= fminf(tmax, r2);
+}
+
+// Conservative ray-segment / box test (boxes are padded at build time).
+__device__ __forceinline__ bool box_hit(const float4 lo, const float4 hi, const Ray& r, float ix, float iy, float iz,
+                                        float ta, float tb) {
+    const float ax = (lo.x - r.ox) * ix, bx = (hi.x - r.ox) * ix;
+    const float ay = (lo.y - r.oy) * iy, by = (hi.y - r.oy) * iy;
+    const float az = (lo.z - r.oz) * iz, bz = (hi.z - r.oz) * iz;
+    const float tn = fmaxf(fmaxf(ta, fminf(ax, bx)), fmaxf(fminf(ay, by), fminf(az, bz)));
+    const float tf = fminf(fminf(tb, fmaxf(ax, bx)), fminf(fmaxf(ay, by), fmaxf(az, bz)));
+    return tn <= tf;
+}
+
+// Stackless traversal of the threaded LBVH: visits, in DFS preorder, every leaf whose padded box
+// meets the ray segment [ta, tb].  on_leaf(slot) returns false to stop the traversal.
+template <typename F>
+__device__ __forceinline__ void traverse(const float4* __restrict__ nodes, int total, const Ray& r, float ta, float tb,
+                                         F&& on_leaf) {
+    const float ix = 1.0f / r.dx, iy = 1.0f / r.dy, iz = 1.0f / r.dz;
+    int idx = 0;
+    while (idx < total) {
+        const float4 lo = nodes[2 * idx];
+        const float4 hi = nodes[2 * idx + 1];
+        const int miss = __float_as_int(lo.w);
+        const int leaf = __float_as_int(hi.w);
+        if (miss <= idx) return;  // malformed link: never loop (idx strictly increases)
+        if (box_hit(lo, hi, r, ix, iy, iz, ta, tb)) {
+            if (leaf >= 0) {
+                if (leaf >= ((total + 1) >> 1)) return;  // malformed leaf index: stop
+                if (!on_leaf(leaf)) return;
+                idx = miss;
+            } else {
+                idx = idx + 1;
+            }
+        } else {
+            idx = miss;
+        }
+    }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace rg

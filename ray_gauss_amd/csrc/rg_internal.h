@@ -1,0 +1,120 @@
+// Internal layouts shared by the kernels and the C-ABI layer of the gfx950 ray caster.
+//
+// HBM layout (n Gaussians, V views; "slot" = position in the LBVH's Morton order):
+//   perm[slot] -> original index, iperm[orig] -> slot                     int32
+//   geo[4*slot+0] = (mu.x, mu.y, mu.z, sigma)                             float4   } 64-B record
+//   geo[4*slot+1] = (M'00, M'01, M'02, h)    h = ln|sigma/thr|            float4   } gathered by
+//   geo[4*slot+2] = (M'10, M'11, M'12, 0)    M' rows = col_k(R)/(s_k*d)   float4   } traversal and
+//   geo[4*slot+3] = (M'20, M'21, M'22, 0)    d = sqrt(2h)                 float4   } sampling
+//   brec[2*slot+0] = (s.x, s.y, s.z, d), brec[2*slot+1] = (q.w, q.x, q.y, q.z)   backward only
+//   leafbox[2*slot+0..1] = padded AABB (lo.xyz, -), (hi.xyz, -)
+//   rgb[v*n + slot] = (r, g, b, 0) per-view colour                           float4
+//   nodes[2*p+0] = (lo.xyz, miss), nodes[2*p+1] = (hi.xyz, leaf)  p = DFS preorder index
+//     internal node: left child p+1, right child right[p]; leaf: leaf = slot, -1 otherwise;
+//     miss = index of the first node after p's subtree (= total when none): a threaded tree,
+//     traversed with no stack.
+//   acc[16*slot + ...] gradient accumulators: 0-2 d mu, 3 d sigma, 4-6 d s, 8-11 d q(w,x,y,z)
+//   drgb[v*n + slot] per-view colour gradient (float4)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/raygauss_hip.h"
+
+namespace rg {
+
+constexpr int kAccStride = 16;
+
+struct DevScene {
+    int n;
+    int total_nodes;  // 2n-1 (0 when n == 0)
+    const float4* geo;
+    const float4* brec;
+    const float4* nodes;
+    const float4* rgb;  // base of the per-view tables
+    const float* box;   // scene box, 6 floats (device)
+    const int* perm;
+};
+
+struct DevParams {
+    float dt;
+    int spb;
+    float t_eps;
+    float sigma_thr;
+    float t_near;
+    float t_far;
+    float bg;
+    int max_hits;    // >0: count slabs with more hits (reference max_prim_slice)
+    int seg_slabs;   // initial segment length
+};
+
+struct DevGrad {
+    float* acc;     // 16 floats per slot
+    float4* drgb;   // per view
+};
+
+// Counters (device): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 5 work queue,
+// 6.. scratch
+struct DevStats {
+    unsigned long long* c;
+};
+
+// ---- launchers (preprocess.hip) ---------------------------------------------------------------
+hipError_t launch_init_box(float* box, hipStream_t s);
+hipError_t launch_preprocess(const rg_gaussians& g, const int* iperm, float sigma_thr, float4* geo, float4* brec,
+                             float4* leafbox, float* box, hipStream_t s);
+hipError_t launch_sh_forward(const rg_gaussians& g, const int* iperm, const rg_camera* cams_dev, int num_views,
+                             float4* rgb, hipStream_t s);
+hipError_t launch_finish_grads(const rg_gaussians& g, const int* iperm, const rg_camera* cams_dev, int num_views,
+                               float* acc, float4* drgb, int exact, const rg_grads& out, hipStream_t s);
+hipError_t launch_zero(float* p, int64_t count, hipStream_t s);
+hipError_t launch_export_predicate(int n, const int* iperm, const float4* geo, float* mu, float* Mp, int* valid,
+                                   hipStream_t s);
+
+// ---- launchers (lbvh.hip) ---------------------------------------------------------------------
+struct BvhBuffers {
+    unsigned long long* keys[2];
+    int* vals[2];
+    unsigned int* hist;     // radix histogram, 256 * tiles
+    int* perm;
+    int* iperm;
+    float4* nodes;
+    int* parent;            // preorder
+    int* right;             // preorder
+    int* counter;           // refit flags
+    int* leaf_node;         // slot -> preorder index of its leaf
+    int* k_parent;          // Karras numbering scratch: parent of internal (n-1) and leaves (n)
+    int* k_left;            // Karras: left child code, right child code, range first/last
+    int* k_right;
+    int* k_first;
+    int* k_last;
+};
+hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, BvhBuffers& b, hipStream_t s);
+hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b, hipStream_t s);
+hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s);
+hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s);
+
+// ---- launchers (trace.hip) --------------------------------------------------------------------
+struct RayJob {
+    int64_t num_rays;
+    const int32_t* pixels;     // single-view mode (nullable)
+    const int32_t* ray_view;   // batch mode (nullable)
+    const int32_t* ray_pixel;  // batch mode
+    int view;                  // single-view mode
+    const rg_camera* cams;     // device array
+    int num_views;
+};
+hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth, float* out_rgb,
+                          float* out_depth, float* out_opacity, DevStats st, hipStream_t s);
+hipError_t launch_backward(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth,
+                           const float* ray_colors, const float* dLdC, DevGrad gr, DevStats st, hipStream_t s);
+hipError_t launch_train(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth, int chunk,
+                        const float* target, int loss_kind, float* out_rgb, float* out_loss, DevGrad gr, DevStats st,
+                        hipStream_t s);
+hipError_t launch_reset_counters(DevStats st, hipStream_t s);
+hipError_t launch_debug_hits(const DevScene& sc, const DevParams& p, const RayJob& job, int max_slabs, int max_pairs,
+                             int32_t* pairs, int64_t* counts, float* ro, float* rd, float* te, float* tx,
+                             hipStream_t s);
+
+}  // namespace rg

@@ -1,0 +1,141 @@
+"""Parity of the HIP ray caster with the CPU FP64 oracle on the same seeded float32 inputs.
+
+Tolerances: images max-abs <= 1e-4 over rays the oracle does not flag as fragile (a threshold
+test within rounding distance of its boundary); gradients <= 1e-3 relative (max-abs error over
+max-abs value, per tensor) with the fragile rays' dL/dC set to zero on both sides; hit sets and
+the FP32 ray grid bit-exact.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from ray_gauss_amd import scenes
+from ray_gauss_amd.caster import TuneConfig
+
+from gpu_helpers import gpu_box, max_rel, oracle_forward, prepared_caster, rel_err
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def small(gpu_device):
+    wl = scenes.workload("small4k")
+    cam = wl.cameras[0].launch_record()
+    rc, t = prepared_caster(wl.scene, wl.params, [cam], gpu_device)
+    return wl, cam, rc, t
+
+
+def test_bvh_structure(small):
+    _, _, rc, _ = small
+    assert rc.check_bvh() == 0
+
+
+def test_ray_grid_bit_exact(small):
+    wl, cam, rc, _ = small
+    px = torch.arange(0, 4096, 3, dtype=torch.int32)
+    _, _, ro, rd, te, tx = rc.debug_hits(0, px, 1, 1)
+    o, d = O.rays(cam, px.numpy())
+    bmin, bmax = gpu_box(rc)
+    te_ref, tx_ref = O.ray_interval(cam, bmin, bmax, wl.params.t_near, wl.params.t_far, px.numpy())
+    assert np.array_equal(ro.cpu().numpy().view(np.uint32), o.view(np.uint32))
+    assert np.array_equal(rd.cpu().numpy().view(np.uint32), d.view(np.uint32))
+    assert np.array_equal(te.cpu().numpy().view(np.uint32), te_ref.view(np.uint32))
+    assert np.array_equal(tx.cpu().numpy().view(np.uint32), tx_ref.view(np.uint32))
+
+
+def _hitset_check(rc, wl, px, max_slabs=4096, max_pairs=4096):
+    pairs, counts, ro, rd, te, tx = rc.debug_hits(0, px, max_slabs, max_pairs)
+    mu, Mp, valid = rc.export_predicate()
+    ref_pairs, ref_counts = O.hits_fp32(mu.cpu().numpy(), Mp.cpu().numpy(), valid.cpu().numpy(), ro.cpu().numpy(),
+                                        rd.cpu().numpy(), te.cpu().numpy(), tx.cpu().numpy(), np.float32(wl.params.dt),
+                                        wl.params.samples_per_slab, max_slabs, max_pairs)
+    counts = counts.cpu().numpy()
+    assert counts.max() <= max_pairs
+    assert np.array_equal(counts, ref_counts)
+    gp = pairs.cpu().numpy()
+    for r in range(len(counts)):
+        c = counts[r]
+        got = gp[r, :c]
+        got = got[np.lexsort((got[:, 1], got[:, 0]))]
+        assert np.array_equal(got, ref_pairs[r, :c]), r
+    return int(counts.sum())
+
+
+def test_hit_sets_bit_exact_small(small):
+    wl, _, rc, _ = small
+    px = torch.arange(0, 4096, 5, dtype=torch.int32)
+    total = _hitset_check(rc, wl, px)
+    assert total > 1000
+
+
+def test_forward_parity_small(small):
+    wl, cam, rc, _ = small
+    rgb, depth, opac = rc.forward(0, depth=True, opacity=True)
+    torch.cuda.synchronize()
+    ref = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc))
+    ok = ref["fragile"] == 0
+    assert ok.mean() > 0.95
+    err = np.abs(rgb.cpu().numpy() - ref["rgb"])[ok].max()
+    assert err <= IMG_TOL, err
+    assert np.abs(opac.cpu().numpy() - ref["opacity"])[ok].max() <= IMG_TOL
+    assert np.abs(depth.cpu().numpy() - ref["depth"])[ok].max() <= 1e-3 * max(1.0, ref["depth"].max())
+    st = rc.stats()
+    assert st["slab_hits"] == int(ref["K"].sum()) or abs(st["slab_hits"] - ref["K"].sum()) <= 0.01 * ref["K"].sum()
+
+
+def test_backward_parity_small(small):
+    wl, cam, rc, _ = small
+    rgb, _, _ = rc.forward(0)
+    ref = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc))
+    ok = (ref["fragile"] == 0).astype(np.float64)
+    target = np.random.default_rng(1).uniform(0, 1, (4096, 3))
+    dL = 2.0 * (rgb.cpu().numpy().astype(np.float64) - target) * ok[:, None]
+    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda())
+    g = rc.finish_grads()
+    box = gpu_box(rc)
+    og = O.full_backward(wl.params, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64),
+                         dL)
+    for k in ("positions", "scales", "quats", "densities", "sh"):
+        e = max_rel(g[k].cpu().numpy(), og[k])
+        assert e <= GRAD_TOL, (k, e, rel_err(g[k].cpu().numpy(), og[k]))
+
+
+@pytest.mark.parametrize("depth", [8, 32])
+def test_tuning_invariance(gpu_device, depth):
+    """Hit-buffer depth and segment length change the schedule, not the image."""
+    wl = scenes.workload("small4k")
+    cam = wl.cameras[0].launch_record()
+    rc0, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device, TuneConfig(16, 8, 128))
+    rc1, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device, TuneConfig(depth, 1 if depth == 8 else 64, 128))
+    a = rc0.forward(0)[0].cpu().numpy()
+    b = rc1.forward(0)[0].cpu().numpy()
+    assert np.abs(a - b).max() <= 1e-6
+
+
+def test_blobs_crop_parity(gpu_device):
+    wl = scenes.workload("blobs100k")
+    cam = wl.cameras[0].launch_record()
+    rc, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device)
+    assert rc.check_bvh() == 0
+    px = torch.from_numpy(scenes.crop_pixels(wl.cameras[0], 384, 384, 32, 32))
+    rgb, _, _ = rc.forward(0, px)
+    ref = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc), px.numpy())
+    ok = ref["fragile"] == 0
+    assert ok.mean() > 0.9
+    err = np.abs(rgb.cpu().numpy() - ref["rgb"])[ok].max()
+    assert err <= IMG_TOL, err
+    dL = np.random.default_rng(2).normal(0, 1, (px.shape[0], 3)) * ok[:, None]
+    rc.backward(0, px, rgb, torch.from_numpy(dL.astype(np.float32)).cuda())
+    g = rc.finish_grads()
+    box = gpu_box(rc)
+    og = O.full_backward(wl.params, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64),
+                         dL, px.numpy())
+    for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis"):
+        e = max_rel(g[k].cpu().numpy(), og[k])
+        assert e <= GRAD_TOL, (k, e)
+    _hitset_check(rc, wl, px[::7].contiguous())
