@@ -1,0 +1,230 @@
+"""Benchmark of the gfx950 differentiable ray caster: train rays/s (forward + backward).
+
+A step is one training pass of the hot path over one batch: LBVH refit + per-Gaussian
+preprocessing, per-view SH/SG colour, forward and backward of every ray of every view (L2 loss
+against a fixed synthetic target), and the SH/SG backward.  Inputs are resident in HBM before
+the timed region.
+
+Two launch structures are timed on the same step:
+  batched  -- one persistent fused train kernel over all views (device work queue), the number
+              reported as `value`;
+  per_view -- the plain path: one forward launch and one backward launch per view.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def tile_order_pixels(w, h):
+    import numpy as np
+
+    tx, ty = (w + 7) // 8, (h + 7) // 8
+    t = np.arange(tx * ty * 64)
+    tile, lane = t // 64, t % 64
+    px = (tile % tx) * 8 + lane % 8
+    py = (tile // tx) * 8 + lane // 8
+    ok = (px < w) & (py < h)
+    return (py * w + px)[ok].astype(np.int32)
+
+
+def algorithmic_bytes(R, P):
+    """Per-ray and per-(ray, Gaussian) HBM bytes of a fused forward+backward launch: per ray the
+    colour written / re-read and dL/dC read (36 B); per distinct pair the forward read of the
+    Gaussian (mu, s, q, sigma, rgb: 56 B), the backward re-read (56 B) and the gradient write
+    (14 fp32 = 56 B)."""
+    return 36.0 * R + 168.0 * P
+
+
+def cpu_baseline(wl, threads=None):
+    """The CPU FP64 oracle (OpenMP over rays) on a bounded sample: a 64x64 crop of view 0."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from ray_gauss_amd import scenes
+
+    cam_obj = wl.cameras[0]
+    cam = cam_obj.launch_record()
+    w, h = cam["width"], cam["height"]
+    px = scenes.crop_pixels(cam_obj, w // 2 - 32, h // 2 - 32, 64, 64)
+    n_threads = O.nthreads(threads)
+    rgb = O.sh_forward(wl.scene, cam["eye"], n_threads)
+    bmin, bmax = O.scene_box(wl.scene, wl.params.sigma_thr)
+    t0 = time.perf_counter()
+    out = O.render(wl.params, wl.scene, rgb, bmin, bmax, cam, px, n_threads)
+    target = np.random.default_rng(0).uniform(0, 1, out["rgb"].shape)
+    O.backward(wl.params, wl.scene, rgb, bmin, bmax, cam, out["rgb"], 2 * (out["rgb"] - target), px, n_threads)
+    dt = time.perf_counter() - t0
+    return {"value": len(px) / dt, "unit": "rays/s", "cores": n_threads, "kind": "port",
+            "sample": f"64x64 crop ({len(px)} rays) of view 0, forward+backward incl. per-call CPU BVH build, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="unbounded1m_8views")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-view", action="store_true")
+    ap.add_argument("--hit-depth", type=int, default=16)
+    ap.add_argument("--seg-slabs", type=int, default=8)
+    ap.add_argument("--chunk", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is design-only "
+                                   "(one replica per GPU, gradient all-reduce over RCCL)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from ray_gauss_amd import scenes
+    from ray_gauss_amd.caster import RayCaster, TuneConfig
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload(args.workload)
+    sc = wl.scene
+    cams = [c.launch_record() for c in wl.cameras]
+    t = {k: torch.from_numpy(np.ascontiguousarray(getattr(sc, k))).to(dev) for k in
+         ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis")}
+    L = sc.num_sg
+
+    # ray batch: every pixel of every view (or the workload's subsets), 8x8 tiles for coherence
+    views, pixels = [], []
+    for v, (c, px) in enumerate(zip(cams, wl.pixels)):
+        p = tile_order_pixels(c["width"], c["height"]) if px is None else np.asarray(px, np.int32)
+        pixels.append(p)
+        views.append(np.full(len(p), v, np.int32))
+    ray_view = torch.from_numpy(np.concatenate(views)).to(dev)
+    ray_pixel = torch.from_numpy(np.concatenate(pixels)).to(dev)
+    R = int(ray_view.shape[0])
+    gen = torch.Generator(device=dev).manual_seed(0)
+    target = torch.rand((R, 3), generator=gen, device=dev)
+    targets_pv = [target[int(sum(len(p) for p in pixels[:v])):][:len(pixels[v])] for v in range(len(cams))]
+    px_dev = [torch.from_numpy(p).to(dev) for p in pixels]
+
+    tune = TuneConfig(args.hit_depth, args.seg_slabs, args.chunk)
+    rc = RayCaster(wl.params, tune)
+
+    def prepare(rebuild=False):
+        rc.prepare(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"], t["sg"] if L else None,
+                   t["sharpness"] if L else None, t["lobe_axis"] if L else None, rebuild=rebuild)
+
+    prepare(rebuild=True)
+    rc.set_views(cams)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    kernel_ms = []
+
+    def step_batched(record=False):
+        prepare()
+        rc.set_views(cams)
+        if record:
+            ev[0].record()
+        rc.train_batch(ray_view, ray_pixel, target, 0, None, loss)
+        if record:
+            ev[1].record()
+        rc.finish_grads()
+
+    def step_per_view():
+        prepare()
+        rc.set_views(cams)
+        for v in range(len(cams)):
+            rgb, _, _ = rc.forward(v, px_dev[v])
+            dL = 2.0 * (rgb - targets_pv[v])
+            rc.backward(v, px_dev[v], rgb, dL)
+        rc.finish_grads()
+
+    def timed(fn, steps, warmup, record=False):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn(record=True) if record else fn()
+            if record:
+                kms.append(ev)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        return dt, kms
+
+    # ---- batched (fused persistent train kernel) ----
+    for _ in range(args.warmup):
+        step_batched()
+    torch.cuda.synchronize()
+    per_kernel = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step_batched(record=True)
+        torch.cuda.synchronize()
+        per_kernel.append(ev[0].elapsed_time(ev[1]))
+    dt_b = (time.perf_counter() - t0) / args.steps
+    st = rc.stats()
+    k_ms = float(np.mean(per_kernel))
+
+    # ---- per-view plain launches ----
+    dt_pv = None
+    if not args.no_per_view:
+        for _ in range(args.warmup):
+            step_per_view()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step_per_view()
+        torch.cuda.synchronize()
+        dt_pv = (time.perf_counter() - t0) / args.steps
+
+    P, K = st["pairs"], st["slab_hits"]
+    B = algorithmic_bytes(R, P)
+    achieved = B / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "train rays/s (fwd+bwd) at 1M Gaussians, batched vs per-view; HBM GB/s",
+        "value": R / dt_b,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_b * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "gaussians": sc.n, "views": len(cams),
+                   "sizes": sorted({(c["width"], c["height"]) for c in cams})[:4], "rays_per_step": R,
+                   "sh_degree": sc.sh_degree, "sg_lobes": L, "dt": wl.params.dt, "t_eps": wl.params.t_eps,
+                   "hit_depth": tune.hit_depth, "seg_slabs": tune.seg_slabs, "chunk": tune.chunk},
+        "per_view_rays_per_s": (R / dt_pv) if dt_pv else None,
+        "batched_over_per_view": (dt_pv / dt_b) if dt_pv else None,
+        "counters": {"K_slab_hits": K, "P_pairs": P, "fallback_slabs": st["fallback_slabs"],
+                     "gathers": st["gathers"]},
+        "roofline": {"bound": "hbm", "kernel": "k_train (fused forward+loss+backward)",
+                     "kernel_ms": k_ms, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes": B},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(wl)
+        except Exception as e:  # pragma: no cover
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -7,21 +7,16 @@ Public surface:
   scenes, cameras                           -- synthetic workloads and pinhole cameras
 """
 
+import importlib
+
 from .scenes import RenderParams  # noqa: F401
+
+_LAZY = {"RenderOptixSettings": "render", "RenderOptixFunction": "render", "render": "render",
+         "RayCaster": "caster", "TuneConfig": "caster"}
 
 
 def __getattr__(name):
     # torch-dependent pieces load lazily so the numpy-only helpers import without a GPU stack
-    if name in ("RenderOptixSettings", "RenderOptixFunction", "render"):
-        from . import render as _r
-
-        return getattr(_r, name)
-    if name in ("RayCaster", "TuneConfig"):
-        from . import caster as _c
-
-        return getattr(_c, name)
-    if name == "optix_utils":
-        from . import optix_utils as _o
-
-        return _o
+    if name in _LAZY:
+        return getattr(importlib.import_module("." + _LAZY[name], __name__), name)
     raise AttributeError(name)
