@@ -83,7 +83,19 @@ def main():
     ap.add_argument("--hit-depth", type=int, default=16)
     ap.add_argument("--seg-slabs", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=128)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "copies": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is design-only "
                                    "(one replica per GPU, gradient all-reduce over RCCL)", "n_gpus": args.gpus}))

@@ -483,26 +483,54 @@ inline bool near_rel(double a, double b, double rel) {
     return std::fabs(a - b) <= rel * std::max(std::fabs(a), std::fabs(b));
 }
 
+// Fragility: a ray is flagged when some threshold decision of the reference's algorithm lies
+// within FP32 rounding of its boundary, so that an FP32 evaluation may legitimately decide it
+// the other way.  The decisions that change the image are the sample tests sigma*w > thr; a
+// hit-set decision (disc vs 1e-7, interval end vs slab end) only matters through the samples of
+// that Gaussian in that slab, and every such sample then has sigma*w at the threshold.  So the
+// oracle also evaluates, without accumulating, the samples of "near" Gaussians -- predicate
+// just failing (disc within kEpsD below 1e-7) or interval just missing the slab (within
+// kEpsT) -- and flags the ray when any sample of a hit or near Gaussian has
+// |sigma*w/thr - 1| <= kEpsW.
+const double kEpsW = 1e-4;  // relative, on sigma*w
+const double kEpsD = 5e-5;  // absolute, on disc
+inline double eps_t(double t) { return 1e-6 * std::max(1.0, std::fabs(t)); }
+
 struct Hit {
     int64_t g;
     double r1, r2;
+    bool near;  // predicate fails by less than kEpsD: evaluated for fragility only
 };
 
-// Collects every Gaussian whose predicate interval overlaps [tenter, texit], sorted by (r1, g).
-// Marks the ray fragile when a predicate test sits on its decision boundary.
+// Collects every Gaussian whose predicate interval overlaps [tenter, texit] (and the near
+// misses), sorted by (r1, g).
 void gather(const CpuBVH& B, const std::vector<Geo>& geo, const double o[3], const double d[3], double t0,
             double t1, std::vector<Hit>& hits, std::vector<int64_t>& scratch, int* fragile) {
     hits.clear();
-    bvh_query(B, o, d, t0, t1, scratch);
+    bvh_query(B, o, d, t0 - eps_t(t0), t1 + eps_t(t1), scratch);
     for (int64_t g : scratch) {
         const Interval I = intersect64(geo[g], o, d);
-        // decision-boundary checks: disc vs 1e-7; the b>0 branch when the origin is inside.
-        if (std::fabs(I.disc - 1e-7) < 2e-6) *fragile = 1;
+        // the b > 0 branch decides hit / no hit when the origin is inside the ellipsoid
         if (I.c < 0.0 && std::fabs(I.b) < 1e-5) *fragile = 1;
-        if (!I.valid) continue;
-        if (near_rel(I.r2, t0, 1e-5) || near_rel(I.r1, t1, 1e-5)) *fragile = 1;
-        if (!overlap(t0, t1, I.r1, I.r2)) continue;
-        hits.push_back({g, I.r1, I.r2});
+        Hit h{g, I.r1, I.r2, false};
+        if (!I.valid) {
+            if (!(geo[g].valid && I.disc > 1e-7 - kEpsD && I.b > 0.0)) continue;
+            // near miss: interval of the ellipsoid it grazes
+            const double sd = std::sqrt(std::max(I.disc, 0.0));
+            const double q = I.b + sd;
+            double l = 0.0;
+            {
+                double Dp[3];
+                for (int k = 0; k < 3; ++k)
+                    Dp[k] = geo[g].Mp[k][0] * d[0] + geo[g].Mp[k][1] * d[1] + geo[g].Mp[k][2] * d[2];
+                l = 1.0 / std::sqrt(Dp[0] * Dp[0] + Dp[1] * Dp[1] + Dp[2] * Dp[2]);
+            }
+            h.r1 = (I.b - sd) * l;
+            h.r2 = q * l;
+            h.near = true;
+        }
+        if (!overlap(t0 - eps_t(t0), t1 + eps_t(t1), h.r1, h.r2)) continue;
+        hits.push_back(h);
     }
     std::sort(hits.begin(), hits.end(), [](const Hit& a, const Hit& b) {
         return a.r1 < b.r1 || (a.r1 == b.r1 && a.g < b.g);
@@ -510,27 +538,29 @@ void gather(const CpuBVH& B, const std::vector<Geo>& geo, const double o[3], con
 }
 
 // Slab iterator shared by forward and backward: produces, per slab, the hit set (indices into
-// `hits`) in ascending Gaussian order.
+// `hits`) in ascending Gaussian order, and the near set (entries that miss the slab's hit set
+// by less than the rounding margins).
 struct SlabWalker {
     const std::vector<Hit>* hits;
     size_t next = 0;
     std::vector<int> active;  // indices into hits
     std::vector<int> cur;
+    std::vector<int> near;
     void step(double tmin, double tmax, int* fragile) {
+        (void)fragile;
         const std::vector<Hit>& H = *hits;
-        while (next < H.size() && H[next].r1 <= tmax) active.push_back((int)next++);
-        // boundary checks for the next not-yet-entered interval
-        if (next < H.size() && near_rel(H[next].r1, tmax, 1e-5)) *fragile = 1;
+        const double emin = eps_t(tmin), emax = eps_t(tmax);
+        while (next < H.size() && H[next].r1 <= tmax + emax) active.push_back((int)next++);
         size_t w = 0;
-        for (size_t a = 0; a < active.size(); ++a) {
-            const Hit& h = H[active[a]];
-            if (near_rel(h.r2, tmin, 1e-5) || near_rel(h.r1, tmax, 1e-5)) *fragile = 1;
-            if (h.r2 >= tmin) active[w++] = active[a];
-        }
+        for (size_t a = 0; a < active.size(); ++a)
+            if (H[active[a]].r2 >= tmin - emin) active[w++] = active[a];
         active.resize(w);
         cur.clear();
-        for (int a : active)
-            if (overlap(tmin, tmax, H[a].r1, H[a].r2)) cur.push_back(a);
+        near.clear();
+        for (int a : active) {
+            if (!H[a].near && overlap(tmin, tmax, H[a].r1, H[a].r2)) cur.push_back(a);
+            else if (overlap(tmin - emin, tmax + emax, H[a].r1, H[a].r2)) near.push_back(a);
+        }
         std::sort(cur.begin(), cur.end(), [&](int a, int b) { return H[a].g < H[b].g; });
     }
     bool exhausted() const { return next >= hits->size() && active.empty(); }
@@ -556,8 +586,18 @@ inline double gauss_w(const Geo& G, const double x[3], double y[3]) {
 // computeBufferForward (forward/gaussians_aabb.cu:437-498) for one slab, FP64.
 void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const double* rgb,
                      const std::vector<Hit>& H, const std::vector<int>& cur, const Ray32& r, float tb,
-                     SampleBuf& buf, int* fragile, int64_t* nc = nullptr) {
+                     SampleBuf& buf, int* fragile, int64_t* nc = nullptr, const std::vector<int>* near = nullptr) {
     const double thr = p.sigma_thr;
+    if (near)
+        for (int a : *near) {
+            const Geo& G = geo[H[a].g];
+            for (int i = 0; i < p.spb; ++i) {
+                const double t = sample_t32(tb, i, p.dt);
+                double x[3], y[3];
+                for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
+                if (G.sigma * gauss_w(G, x, y) >= thr * (1.0 - kEpsW)) *fragile = 1;
+            }
+        }
     for (int a : cur) {
         const Geo& G = geo[H[a].g];
         for (int i = 0; i < p.spb; ++i) {
@@ -566,7 +606,7 @@ void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const dou
             for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
             const double w = gauss_w(G, x, y);
             const double sw = G.sigma * w;
-            if (near_rel(sw, thr, 2e-5)) *fragile = 1;
+            if (std::fabs(sw / thr - 1.0) <= kEpsW) *fragile = 1;
             if (sw > thr) {
                 if (nc) ++*nc;
                 buf.sig[i] += sw;
@@ -623,6 +663,11 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
         out.slabs++;
         if (tmax > c.tenter) {
             W.step(tmin, tmax, &out.fragile);
+            if (W.cur.empty() && !W.near.empty()) {
+                buf.reset(p.spb);
+                const std::vector<int> none;
+                accumulate_slab(p, geo, rgb, hits, none, c.r, tb, buf, &out.fragile, nullptr, &W.near);
+            }
             if (!W.cur.empty()) {
                 out.K += (int64_t)W.cur.size();
                 out.maxhits = std::max<int64_t>(out.maxhits, (int64_t)W.cur.size());
@@ -632,7 +677,7 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
                         out.P++;
                     }
                 buf.reset(p.spb);
-                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, tb, buf, &out.fragile, &out.nc);
+                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, tb, buf, &out.fragile, &out.nc, &W.near);
                 for (int i = 0; i < p.spb; ++i) {
                     const double s = buf.sig[i];
                     const double alpha = 1.0 - std::exp(-s * dt);

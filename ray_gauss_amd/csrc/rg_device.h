@@ -74,6 +74,7 @@ struct Isect {
     float Cq;      // h * |D'|^2
     float Px, Py, Pz;     // O' + b D^   (ellipsoid-space offset at closest approach)
     float Dx, Dy, Dz;     // D' = M' d
+    float l;              // 1 / |D'|
 };
 
 // __intersection__gaussian (reference cuda_train/forward/gaussians_aabb.cu:324-402), exact FP32
@@ -112,7 +113,32 @@ __device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, cons
     I.Dx = Dp0;
     I.Dy = Dp1;
     I.Dz = Dp2;
+    I.l = l;
     return true;
+}
+
+// Sampling set-up re-centred at the closest approach.  b = -O'.D^ is a difference of
+// ellipsoid-space quantities of size |o - mu| / s', so t_c = b l and P = O' + b D^ carry an
+// absolute error that grows with the distance to the camera.  Evaluating O' again at
+// x0 = o + t_c d (a point inside the ellipsoid) leaves only the rounding of x0 itself, like the
+// reference's per-sample x = o + t d.  Does not touch the hit decision (r1, r2).
+__device__ __forceinline__ void refine_sampling(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
+                                                const Ray& r, Isect& I) {
+    const float t0 = I.tc;
+    const float X0 = (r.ox + t0 * r.dx) - g0.x;
+    const float X1 = (r.oy + t0 * r.dy) - g0.y;
+    const float X2 = (r.oz + t0 * r.dz) - g0.z;
+    const float O0 = g1.x * X0 + g1.y * X1 + g1.z * X2;
+    const float O1 = g2.x * X0 + g2.y * X1 + g2.z * X2;
+    const float O2 = g3.x * X0 + g3.y * X1 + g3.z * X2;
+    const float Dh0 = I.Dx * I.l, Dh1 = I.Dy * I.l, Dh2 = I.Dz * I.l;
+    const float b = -(O0 * Dh0 + O1 * Dh1 + O2 * Dh2);
+    const float P0 = O0 + b * Dh0, P1 = O1 + b * Dh1, P2 = O2 + b * Dh2;
+    I.tc = t0 + b * I.l;
+    I.A = g1.w * (P0 * P0 + P1 * P1 + P2 * P2);
+    I.Px = P0;
+    I.Py = P1;
+    I.Pz = P2;
 }
 
 __device__ __forceinline__ bool overlaps(float tmin, float tmax, float r1, float r2) {

@@ -384,6 +384,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
             load_rec(sc.geo, slot, g0, g1, g2, g3);
             Isect I;
             intersect(g0, g1, g2, g3, ray, I);
+            refine_sampling(g0, g1, g2, g3, ray, I);
             eval_hit(slot, I, tb, tmin, sig, cr, cg, cb);
             ++nh;
         }
@@ -400,6 +401,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
             Isect I;
             if (!intersect(g0, g1, g2, g3, ray, I)) return true;
             if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+            refine_sampling(g0, g1, g2, g3, ray, I);
             eval_hit(slot, I, tb, tmin, sig, cr, cg, cb);
             ++nh;
             return true;
@@ -429,6 +431,7 @@ __device__ void backward_ray(const DevScene& sc, const DevParams& p, const Ray& 
             load_rec(sc.geo, slot, g0, g1, g2, g3);
             Isect I;
             intersect(g0, g1, g2, g3, ray, I);
+            refine_sampling(g0, g1, g2, g3, ray, I);
             accum_hit<SPB>(I, g0.w, rgbv[slot], tb, p, sig, cr, cg, cb);
             ++nh;
         }
@@ -442,6 +445,7 @@ __device__ void backward_ray(const DevScene& sc, const DevParams& p, const Ray& 
             load_rec(sc.geo, slot, g0, g1, g2, g3);
             Isect I;
             intersect(g0, g1, g2, g3, ray, I);
+            refine_sampling(g0, g1, g2, g3, ray, I);
             grad_hit<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], tb, p, sig, cr, cg, dL, sc, gr, view);
         }
     };
@@ -456,6 +460,7 @@ __device__ void backward_ray(const DevScene& sc, const DevParams& p, const Ray& 
             Isect I;
             if (!intersect(g0, g1, g2, g3, ray, I)) return true;
             if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+            refine_sampling(g0, g1, g2, g3, ray, I);
             accum_hit<SPB>(I, g0.w, rgbv[slot], tb, p, sig, cr, cg, cb);
             ++nh;
             return true;
@@ -468,6 +473,7 @@ __device__ void backward_ray(const DevScene& sc, const DevParams& p, const Ray& 
             Isect I;
             if (!intersect(g0, g1, g2, g3, ray, I)) return true;
             if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+            refine_sampling(g0, g1, g2, g3, ray, I);
             grad_hit<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], tb, p, sig, cr, cg, dL, sc, gr, view);
             return true;
         });
