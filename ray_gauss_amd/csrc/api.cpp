@@ -202,11 +202,11 @@ int rg_create(rg_handle** out, const rg_config* cfg) {
         return rc;
     }
     h->cfg = *cfg;
-    if ((rc = dalloc(h, &h->box, 8)) || (rc = dalloc(h, &h->counters, 8))) {
+    if ((rc = dalloc(h, &h->box, 8)) || (rc = dalloc(h, &h->counters, 16))) {
         delete h;
         return rc;
     }
-    if (hipMemset(h->counters, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMemset(h->counters, 0, 16 * sizeof(unsigned long long)) != hipSuccess) {
         delete h;
         return RG_ERR_HIP;
     }

@@ -54,8 +54,8 @@ struct DevGrad {
     float4* drgb;   // per view
 };
 
-// Counters (device): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 5 work queue,
-// 6.. scratch
+// Counters (device, 16): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 6 loss (float),
+// 7 debug scratch, 8..15 per-XCD work queues
 struct DevStats {
     unsigned long long* c;
 };

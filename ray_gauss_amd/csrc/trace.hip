@@ -30,7 +30,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kMaxSeg = 4096;
 
-enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_QUEUE = 5, C_LOSS = 6 };
+enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6, C_QUEUE0 = 8 };
 
 struct RayOut {
     float C[3];
@@ -564,15 +564,24 @@ __global__ void __launch_bounds__(kWave) k_train(DevScene sc, DevParams p, RayJo
     const int lane = threadIdx.x;
     Counters ctr;
     float loss = 0.0f;
-    while (true) {
+    // The work items are split into 8 contiguous ranges with one queue each; a wave drains the
+    // queue of the XCD it runs on first (that XCD's L2 then serves a compact part of the scene)
+    // and then steals from the others.  Every queue is drained, so the XCD id only affects speed.
+    const int xcc = (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 7);
+    const int64_t per = (num_items + 7) / 8;
+    for (int k = 0; k < 8; ++k) {
+      const int qid = (xcc + k) & 7;
+      const int64_t q0 = (int64_t)qid * per;
+      const int64_t q1 = min(num_items, q0 + per);
+      while (q0 < q1) {
         int64_t base = 0;
-        if (lane == 0) base = (int64_t)atomicAdd(&st.c[C_QUEUE], (unsigned long long)chunk);
+        if (lane == 0) base = (int64_t)atomicAdd(&st.c[C_QUEUE0 + qid], (unsigned long long)chunk);
         base = __shfl(base, 0, 64);
-        if (base >= num_items) break;
+        if (q0 + base >= q1) break;
         for (int j = 0; j < chunk; j += kWave) {
-            const int64_t item = base + j + lane;
+            const int64_t item = q0 + base + j + lane;
             RayRef q;
-            if (item < num_items && resolve(job, item, q)) {
+            if (item < q1 && resolve(job, item, q)) {
                 const rg_camera cam = job.cams[q.view];
                 const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
                 const float4* rgbv = sc.rgb + (size_t)q.view * sc.n;
@@ -602,6 +611,7 @@ __global__ void __launch_bounds__(kWave) k_train(DevScene sc, DevParams p, RayJo
                 }
             }
         }
+      }
     }
     flush_counters(ctr, st);
     const float ls = wave_sum(loss);
@@ -654,7 +664,7 @@ __global__ void __launch_bounds__(kWave) k_debug_hits(DevScene sc, DevParams p, 
 }
 
 __global__ void k_reset_counters(unsigned long long* c) {
-    if (threadIdx.x < 8) c[threadIdx.x] = 0;
+    if (threadIdx.x < 16) c[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------------ dispatch -----------------
