@@ -48,6 +48,8 @@ typedef struct rg_config {
     int32_t seg_slabs;         /* initial slabs per interval gather: 1..4096                     */
     int32_t chunk;             /* rays per work item in the persistent kernels: 64, 128, 256    */
     int32_t max_hits_per_slab; /* >0: count slabs whose hit set exceeds it (max_prim_slice)      */
+    int32_t xcd_local_grads;   /* 1: one gradient copy per XCD (L2-local atomics), summed at the
+                                  end; 0: one copy, device-scope atomics                          */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

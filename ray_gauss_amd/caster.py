@@ -20,11 +20,13 @@ from .cameras import launch_record_from
 
 @dataclass
 class TuneConfig:
-    """Launch tuning: interval-list depth per ray, initial slabs per gather, rays per queue item."""
+    """Launch tuning: interval-list depth per ray, initial slabs per gather, rays per queue item,
+    and whether gradients accumulate in one copy per XCD (L2-local atomics)."""
 
     hit_depth: int = 16
     seg_slabs: int = 8
     chunk: int = 128
+    xcd_grads: bool = True
 
 
 def _ptr(t):
@@ -102,6 +104,7 @@ class RayCaster:
         c.seg_slabs = int(self.tune.seg_slabs)
         c.chunk = int(self.tune.chunk)
         c.max_hits_per_slab = int(self.max_hits_per_slab)
+        c.xcd_local_grads = int(bool(self.tune.xcd_grads))
         return c
 
     def set_tune(self, tune: TuneConfig):

@@ -40,6 +40,8 @@ struct rg_handle {
 
 namespace {
 
+constexpr int kGradCopies = 8;  // one gradient copy per XCD
+
 int fail(rg_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
     return code;
@@ -88,7 +90,7 @@ int ensure_gaussian_capacity(rg_handle* h, int64_t n) {
     if ((rc = dalloc(h, &h->geo, 4 * cap))) return rc;
     if ((rc = dalloc(h, &h->brec, 2 * cap))) return rc;
     if ((rc = dalloc(h, &h->leafbox, 2 * cap))) return rc;
-    if ((rc = dalloc(h, &h->acc, rg::kAccStride * cap))) return rc;
+    if ((rc = dalloc(h, &h->acc, (int64_t)kGradCopies * rg::kAccStride * cap))) return rc;
     rg::BvhBuffers& b = h->bvh;
     if ((rc = dalloc(h, &b.keys[0], cap))) return rc;
     if ((rc = dalloc(h, &b.keys[1], cap))) return rc;
@@ -122,9 +124,9 @@ int ensure_view_capacity(rg_handle* h, int views) {
     }
     if (need > h->rgb_cap || !h->rgb) {
         if ((rc = dalloc(h, &h->rgb, need))) return rc;
-        if ((rc = dalloc(h, &h->drgb, need))) return rc;
+        if ((rc = dalloc(h, &h->drgb, kGradCopies * need))) return rc;
         h->rgb_cap = need;
-        HIPCHK(h, hipMemset(h->drgb, 0, sizeof(float4) * need));
+        HIPCHK(h, hipMemset(h->drgb, 0, sizeof(float4) * kGradCopies * need));
     }
     return RG_OK;
 }
@@ -179,6 +181,16 @@ rg::RayJob single_job(rg_handle* h, int view, int64_t num_rays, const int32_t* p
         j.num_rays = (int64_t)((c.width + 7) / 8) * ((c.height + 7) / 8) * 64;
     }
     return j;
+}
+
+rg::DevGrad dev_grad(rg_handle* h) {
+    rg::DevGrad g;
+    g.acc = h->acc;
+    g.drgb = h->drgb;
+    g.copies = h->cfg.xcd_local_grads ? kGradCopies : 1;
+    g.acc_stride = (int64_t)rg::kAccStride * h->cap;
+    g.drgb_stride = h->rgb_cap;
+    return g;
 }
 
 rg::DevStats dev_stats(rg_handle* h) {
@@ -260,7 +272,7 @@ int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* strea
         HIPCHK(h, rg::launch_iota(n, h->bvh.perm, h->bvh.iperm, s));
         HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
         HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
-        HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
+        HIPCHK(h, rg::launch_zero(h->acc, (int64_t)kGradCopies * rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
     }
     HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
     HIPCHK(h, rg::launch_lbvh_refit(n, h->leafbox, h->bvh, s));
@@ -316,7 +328,7 @@ int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pix
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
-    rg::DevGrad gr{h->acc, h->drgb};
+    const rg::DevGrad gr = dev_grad(h);
     HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, ray_colors, dL_dC, gr,
                                   dev_stats(h), s));
     return RG_OK;
@@ -358,7 +370,7 @@ int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colo
     if ((rc = batch_job(h, b, j))) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
-    rg::DevGrad gr{h->acc, h->drgb};
+    const rg::DevGrad gr = dev_grad(h);
     HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, ray_colors, dL_dC, gr,
                                   dev_stats(h), s));
     return RG_OK;
@@ -373,7 +385,7 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
     if ((rc = batch_job(h, b, j))) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
-    rg::DevGrad gr{h->acc, h->drgb};
+    const rg::DevGrad gr = dev_grad(h);
     HIPCHK(h, rg::launch_train(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, h->cfg.chunk, target, loss_kind,
                                out_rgb, out_loss, gr, dev_stats(h), s));
     if (out_loss)
@@ -384,8 +396,8 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
 int rg_zero_grads(rg_handle* h, void* stream) {
     if (!h) return RG_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->acc) HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * h->cap, s));
-    if (h->drgb) HIPCHK(h, rg::launch_zero(reinterpret_cast<float*>(h->drgb), 4 * h->rgb_cap, s));
+    if (h->acc) HIPCHK(h, rg::launch_zero(h->acc, (int64_t)kGradCopies * rg::kAccStride * h->cap, s));
+    if (h->drgb) HIPCHK(h, rg::launch_zero(reinterpret_cast<float*>(h->drgb), 4 * kGradCopies * h->rgb_cap, s));
     return RG_OK;
 }
 
@@ -397,7 +409,7 @@ int rg_finish_grads(rg_handle* h, const rg_grads* out, void* stream) {
     if (h->n > 0 && h->g.num_sg > 0 && (!out->sg || !out->sharpness || !out->lobe_axis))
         return fail(h, RG_ERR_ARG, "null SG gradient array with num_sg > 0");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    HIPCHK(h, rg::launch_finish_grads(h->g, h->bvh.iperm, h->cams, h->num_views, h->acc, h->drgb,
+    HIPCHK(h, rg::launch_finish_grads(h->g, h->bvh.iperm, h->cams, h->num_views, dev_grad(h),
                                       h->cfg.exact_sh_grad, *out, s));
     return RG_OK;
 }

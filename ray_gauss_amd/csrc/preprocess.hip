@@ -201,8 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(int n, int deg, int L, const 
                                                    const float* __restrict__ sh, const float* __restrict__ sg,
                                                    const float* __restrict__ lam, const float* __restrict__ axis,
                                                    const int* __restrict__ iperm, const rg_camera* __restrict__ cams,
-                                                   int num_views, float* __restrict__ acc,
-                                                   float4* __restrict__ drgb, int exact, rg_grads out) {
+                                                   int num_views, DevGrad gr, int exact, rg_grads out) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     if (g >= n) return;
     const int K = (deg + 1) * (deg + 1);
@@ -222,9 +221,15 @@ __global__ void __launch_bounds__(kBlock) k_finish(int n, int deg, int L, const 
     const float mx = pos[3 * g], my = pos[3 * g + 1], mz = pos[3 * g + 2];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
     for (int v = 0; v < num_views; ++v) {
-        float4 dl4 = drgb[(size_t)v * n + slot];
-        drgb[(size_t)v * n + slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float dL[3] = {dl4.x, dl4.y, dl4.z};
+        float dL[3] = {0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < gr.copies; ++c) {
+            float4* p = gr.drgb + (size_t)c * gr.drgb_stride + (size_t)v * n + slot;
+            const float4 d4 = *p;
+            *p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            dL[0] += d4.x;
+            dL[1] += d4.y;
+            dL[2] += d4.z;
+        }
         if (dL[0] == 0.0f && dL[1] == 0.0f && dL[2] == 0.0f) continue;  // sh_utils.cu:129-131
         const rg_camera& cam = cams[v];
         const float d0x = mx - cam.eye[0], d0y = my - cam.eye[1], d0z = mz - cam.eye[2];
@@ -300,14 +305,18 @@ __global__ void __launch_bounds__(kBlock) k_finish(int n, int deg, int L, const 
             for (int k = 0; k < 3; ++k) out.lobe_axis[(size_t)g * 3 * L + 3 * l + k] = inr ? dax[l][k] : 0.0f;
         }
     }
-    float* a = acc + (size_t)kAccStride * slot;
-    const float4 a0 = reinterpret_cast<float4*>(a)[0];
-    const float4 a1 = reinterpret_cast<float4*>(a)[1];
-    const float4 a2 = reinterpret_cast<float4*>(a)[2];
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    reinterpret_cast<float4*>(a)[0] = z4;
-    reinterpret_cast<float4*>(a)[1] = z4;
-    reinterpret_cast<float4*>(a)[2] = z4;
+    float4 a0 = z4, a1 = z4, a2 = z4;
+    for (int c = 0; c < gr.copies; ++c) {
+        float4* a = reinterpret_cast<float4*>(gr.acc + (size_t)c * gr.acc_stride + (size_t)kAccStride * slot);
+        const float4 b0 = a[0], b1 = a[1], b2 = a[2];
+        a[0] = z4;
+        a[1] = z4;
+        a[2] = z4;
+        a0 = make_float4(a0.x + b0.x, a0.y + b0.y, a0.z + b0.z, a0.w + b0.w);
+        a1 = make_float4(a1.x + b1.x, a1.y + b1.y, a1.z + b1.z, a1.w + b1.w);
+        a2 = make_float4(a2.x + b2.x, a2.y + b2.y, a2.z + b2.z, a2.w + b2.w);
+    }
     out.positions[3 * g + 0] = a0.x + dpx;
     out.positions[3 * g + 1] = a0.y + dpy;
     out.positions[3 * g + 2] = a0.z + dpz;
@@ -365,11 +374,11 @@ hipError_t launch_sh_forward(const rg_gaussians& g, const int* iperm, const rg_c
 }
 
 hipError_t launch_finish_grads(const rg_gaussians& g, const int* iperm, const rg_camera* cams_dev, int num_views,
-                               float* acc, float4* drgb, int exact, const rg_grads& out, hipStream_t s) {
+                               const DevGrad& gr, int exact, const rg_grads& out, hipStream_t s) {
     if (g.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finish, dim3(blocks(g.n, kBlock)), dim3(kBlock), 0, s, (int)g.n, g.sh_degree, g.num_sg,
-                       g.positions, g.sh, g.sg, g.sharpness, g.lobe_axis, iperm, cams_dev, num_views, acc, drgb,
-                       exact, out);
+                       g.positions, g.sh, g.sg, g.sharpness, g.lobe_axis, iperm, cams_dev, num_views, gr, exact,
+                       out);
     return hipGetLastError();
 }
 

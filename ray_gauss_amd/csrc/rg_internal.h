@@ -49,9 +49,15 @@ struct DevParams {
     int seg_slabs;   // initial segment length
 };
 
+// Gradient accumulators.  With copies == 8 each XCD adds into its own copy with workgroup-scope
+// atomics, which complete in that XCD's L2; the finish kernel sums the copies.  With copies == 1
+// all waves add into one buffer with device-scope atomics.
 struct DevGrad {
-    float* acc;     // 16 floats per slot
-    float4* drgb;   // per view
+    float* acc;            // copies x (16 floats per slot)
+    float4* drgb;          // copies x (views x slots)
+    int copies;            // 1 or 8
+    int64_t acc_stride;    // floats between copies
+    int64_t drgb_stride;   // float4 between copies
 };
 
 // Counters (device, 16): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 6 loss (float),
@@ -67,7 +73,7 @@ hipError_t launch_preprocess(const rg_gaussians& g, const int* iperm, float sigm
 hipError_t launch_sh_forward(const rg_gaussians& g, const int* iperm, const rg_camera* cams_dev, int num_views,
                              float4* rgb, hipStream_t s);
 hipError_t launch_finish_grads(const rg_gaussians& g, const int* iperm, const rg_camera* cams_dev, int num_views,
-                               float* acc, float4* drgb, int exact, const rg_grads& out, hipStream_t s);
+                               const DevGrad& gr, int exact, const rg_grads& out, hipStream_t s);
 hipError_t launch_zero(float* p, int64_t count, hipStream_t s);
 hipError_t launch_export_predicate(int n, const int* iperm, const float4* geo, float* mu, float* Mp, int* valid,
                                    hipStream_t s);

@@ -50,6 +50,16 @@ struct Counters {
     unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0;
 };
 
+// XCD (accelerator complex die) the wave runs on: hardware register XCC_ID, low 3 bits.
+__device__ __forceinline__ int xcc_id() { return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 7); }
+
+// FP32 gradient add: workgroup scope keeps the atomic in the XCD's L2 (per-XCD copies);
+// device scope otherwise.
+__device__ __forceinline__ void gadd(float* p, float v, bool local) {
+    if (local) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else atomicAdd(p, v);
+}
+
 __device__ __forceinline__ void load_rec(const float4* __restrict__ geo, int s, float4& g0, float4& g1, float4& g2,
                                          float4& g3) {
     g0 = geo[4 * s];
@@ -200,23 +210,25 @@ __device__ __forceinline__ void grad_hit(const Isect& I, int slot, float4 g0, fl
                       4.0f * qj * (c0[0] + c2[2]);
     const float dqk = 2.0f * (r * c0[1] + qi * c0[2] - r * c1[0] + qj * c1[2] + qi * c2[0] + qj * c2[1]) -
                       4.0f * qk * (c0[0] + c1[1]);
-    float* acc = gr.acc + (size_t)kAccStride * slot;
-    atomicAdd(acc + 0, dmu[0]);
-    atomicAdd(acc + 1, dmu[1]);
-    atomicAdd(acc + 2, dmu[2]);
-    atomicAdd(acc + 3, Ss);
-    atomicAdd(acc + 4, Y[0][0] / s[0]);
-    atomicAdd(acc + 5, Y[1][1] / s[1]);
-    atomicAdd(acc + 6, Y[2][2] / s[2]);
-    atomicAdd(acc + 8, dqr);
-    atomicAdd(acc + 9, dqi);
-    atomicAdd(acc + 10, dqj);
-    atomicAdd(acc + 11, dqk);
-    float* dr = reinterpret_cast<float*>(gr.drgb + (size_t)view * sc.n + slot);
+    const bool local = gr.copies > 1;
+    const int copy = local ? xcc_id() : 0;
+    float* acc = gr.acc + (size_t)copy * gr.acc_stride + (size_t)kAccStride * slot;
+    gadd(acc + 0, dmu[0], local);
+    gadd(acc + 1, dmu[1], local);
+    gadd(acc + 2, dmu[2], local);
+    gadd(acc + 3, Ss, local);
+    gadd(acc + 4, Y[0][0] / s[0], local);
+    gadd(acc + 5, Y[1][1] / s[1], local);
+    gadd(acc + 6, Y[2][2] / s[2], local);
+    gadd(acc + 8, dqr, local);
+    gadd(acc + 9, dqi, local);
+    gadd(acc + 10, dqj, local);
+    gadd(acc + 11, dqk, local);
+    float* dr = reinterpret_cast<float*>(gr.drgb + (size_t)copy * gr.drgb_stride + (size_t)view * sc.n + slot);
     const float sc_ = sigma * Sc;
-    atomicAdd(dr + 0, dL[0] * sc_);
-    atomicAdd(dr + 1, dL[1] * sc_);
-    atomicAdd(dr + 2, dL[2] * sc_);
+    gadd(dr + 0, dL[0] * sc_, local);
+    gadd(dr + 1, dL[1] * sc_, local);
+    gadd(dr + 2, dL[2] * sc_, local);
 }
 
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }
@@ -567,7 +579,7 @@ __global__ void __launch_bounds__(kWave) k_train(DevScene sc, DevParams p, RayJo
     // The work items are split into 8 contiguous ranges with one queue each; a wave drains the
     // queue of the XCD it runs on first (that XCD's L2 then serves a compact part of the scene)
     // and then steals from the others.  Every queue is drained, so the XCD id only affects speed.
-    const int xcc = (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 7);
+    const int xcc = xcc_id();
     const int64_t per = (num_items + 7) / 8;
     for (int k = 0; k < 8; ++k) {
       const int qid = (xcc + k) & 7;
