@@ -171,8 +171,16 @@ def test_dropin_autograd_boundary(gpu_device):
     target = torch.rand(img.shape, generator=gen, device=dev)
     loss = ((img - target) ** 2).sum()
     loss.backward()
-    # same gradients through the caster directly (fragility plays no role: same FP32 path)
-    rc, t = prepared_caster(sc, wl.params, [wl.cameras[0].launch_record()], dev)
+    # same image and gradients through the caster directly, on the same activated tensors
+    from ray_gauss_amd.caster import RayCaster
+
+    with torch.no_grad():
+        s_act = torch.exp(log_s)
+        q_act = q_raw / torch.norm(q_raw, dim=1)[:, None]
+        c_act = torch.cat([rgb_dc[:, :, None], sh_rest], dim=2)
+    rc = RayCaster(wl.params)
+    rc.prepare(pos.detach(), s_act, q_act, dens.detach(), c_act, rebuild=True)
+    rc.set_views([wl.cameras[0].launch_record()])
     rgb, _, _ = rc.forward(0)
     assert torch.equal(rgb, img.detach())
     rc.backward(0, None, rgb, 2 * (rgb - target))
@@ -180,9 +188,13 @@ def test_dropin_autograd_boundary(gpu_device):
     assert max_rel(dens.grad.cpu().numpy(), g["densities"].cpu().numpy()) <= 1e-5
     assert max_rel(pos.grad.cpu().numpy(), g["positions"].cpu().numpy()) <= 1e-5
     # chain through exp(log s): d/dlog s = s * d/ds
-    assert max_rel(log_s.grad.cpu().numpy(), (g["scales"] * t["scales"]).cpu().numpy()) <= 1e-5
-    assert q_raw.grad is not None and torch.isfinite(q_raw.grad).all()
-    assert torch.allclose(rgb_dc.grad, g["sh"][:, :, 0], rtol=1e-5, atol=1e-7)
+    assert max_rel(log_s.grad.cpu().numpy(), (g["scales"] * s_act).cpu().numpy()) <= 1e-5
+    # chain through q / |q|: (I - q^ q^T) / |q| applied to d/dq^
+    qn = torch.norm(q_raw.detach(), dim=1, keepdim=True)
+    gq = g["quats"]
+    dq_raw = (gq - q_act * (q_act * gq).sum(1, keepdim=True)) / qn
+    assert max_rel(q_raw.grad.cpu().numpy(), dq_raw.cpu().numpy()) <= 1e-5
+    assert max_rel(rgb_dc.grad.cpu().numpy(), g["sh"][:, :, 0].cpu().numpy()) <= 1e-5
 
 
 def test_dropin_rejects_jitter(gpu_device):
