@@ -124,6 +124,9 @@ __device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, cons
 // reference's per-sample x = o + t d.  Does not touch the hit decision (r1, r2).
 __device__ __forceinline__ void refine_sampling(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
                                                 const Ray& r, Isect& I) {
+#pragma clang fp contract(off)
+    // contraction off: the result must not depend on the inlining context, because the gather
+    // caches it and the backward recomputes it
     const float t0 = I.tc;
     const float X0 = (r.ox + t0 * r.dx) - g0.x;
     const float X1 = (r.oy + t0 * r.dy) - g0.y;

@@ -9,17 +9,25 @@
 // LDS (capacity hit_depth); each slab's hit set is then {entries: max(tmin,r1) <= min(tmax,r2)},
 // the reference's own predicate.  m adapts to the list fill (doubling across empty space,
 // halving on overflow); a slab whose own hit set exceeds the list is processed by a per-slab
-// traversal that accumulates on the fly (no storage).  Fill order is the traversal's DFS order,
-// and forward and backward make identical segment decisions, so both sum each slab in the same
-// order.
+// traversal that accumulates on the fly (no storage).  Forward and backward step through slabs
+// with the same SegState machine, so both make identical segment decisions and sum each slab in
+// the same order (the traversal's DFS order, which is increasing Morton slot).
 //
 // Sampling uses the closed form of the Gaussian along the ray: with the predicate's quantities
 // (t_c = b l, P = O' + b D^, D' = M' d, h = ln|sigma/thr|),
 //     0.5 |M (o + t d - mu)|^2 = h |P|^2 + h |D'|^2 (t - t_c)^2,
 // so each (sample, hit) costs one FMA pair and one exp.  The backward accumulates, per
 // (slab, hit), four scalars (sum w phi, sum w phi tau, sum w phi tau^2, sum w B) and expands
-// them into the 14 parameter gradients of backward/gaussians_aabb.cu:700-763 once.
+// them into the 14 parameter gradients of backward/gaussians_aabb.cu:700-763.
+//
+// Gradient scatter is wave-cooperative: every lane's list is sorted by Morton slot, so after each
+// lane has prepared one slab, the 64 lanes (an 8x8 pixel tile) walk the union of their hit sets in
+// increasing slot order (a wave-wide min per step); the lanes holding the current Gaussian compute
+// their contributions together, the wave sums them, and one lane issues the 14 atomics.
+
 #include <hip/hip_runtime.h>
+
+#include <climits>
 
 #include "rg_device.h"
 #include "rg_internal.h"
@@ -29,6 +37,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kMaxSeg = 4096;
+constexpr int kGradVals = 14;
 
 enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6, C_QUEUE0 = 8 };
 
@@ -38,13 +47,29 @@ struct RayOut {
     float T;
 };
 
-// Per-lane interval list in LDS, [entry][lane] so lanes access consecutive words.
+// Per-lane interval list in LDS, [entry][lane] so lanes access consecutive words: the Gaussian's
+// slot, its predicate interval and its sampling quadratic (t_c, A, Cq) from the gather.
 template <int CAP>
 struct IList {
     int g[CAP][kWave];
     float r1[CAP][kWave];
     float r2[CAP][kWave];
+    float tc[CAP][kWave];
+    float A[CAP][kWave];
+    float Cq[CAP][kWave];
 };
+
+// Sampling quadratic of list entry e (as computed by intersect + refine_sampling at gather time).
+template <int CAP>
+__device__ __forceinline__ Isect entry_isect(const IList<CAP>& L, int e, int lane) {
+    Isect I;
+    I.r1 = L.r1[e][lane];
+    I.r2 = L.r2[e][lane];
+    I.tc = L.tc[e][lane];
+    I.A = L.A[e][lane];
+    I.Cq = L.Cq[e][lane];
+    return I;
+}
 
 struct Counters {
     unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0;
@@ -58,6 +83,21 @@ __device__ __forceinline__ int xcc_id() { return (int)(__builtin_amdgcn_s_getreg
 __device__ __forceinline__ void gadd(float* p, float v, bool local) {
     if (local) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     else atomicAdd(p, v);
+}
+
+__device__ __forceinline__ float wave_sum64(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_min64(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_min64f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
 __device__ __forceinline__ void load_rec(const float4* __restrict__ geo, int s, float4& g0, float4& g1, float4& g2,
@@ -78,7 +118,7 @@ __device__ __forceinline__ void accum_hit(const Isect& I, float sigma, float4 co
         if (SPB > 8 && i >= p.spb) break;
         const float t = sample_t(tb, i, p.dt);
         const float tau = t - I.tc;
-        const float pw = I.A + I.Cq * (tau * tau);
+        const float pw = fmaf(I.Cq, tau * tau, I.A);  // explicit: same rounding in every pass
         const float sw = sigma * __expf(-pw);
         if (sw > p.sigma_thr) {
             sig[i] += sw;
@@ -147,12 +187,11 @@ __device__ __forceinline__ void slab_coeffs(float* sig, float* cr, float* cg, fl
 }
 
 // Gradient of one (slab, hit): four sufficient statistics over the slab's samples, expanded
-// into d mu, d s, d q, d sigma, d rgb and scattered with FP32 atomics.
+// into v = (d mu[3], d sigma, d s[3], d q[4] (w,x,y,z), d rgb[3]).
 template <int SPB>
-__device__ __forceinline__ void grad_hit(const Isect& I, int slot, float4 g0, float4 g1, float4 g2, float4 g3,
-                                         float4 col, float tb, const DevParams& p, const float* Aq, const float* Bq,
-                                         const float* Eq, const float* dL, const DevScene& sc, const DevGrad& gr,
-                                         int view) {
+__device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, float4 g1, float4 g2, float4 g3,
+                                          float4 col, float tb, const DevParams& p, const float* Aq, const float* Bq,
+                                          const float* Eq, const float* dL, const DevScene& sc, float* v) {
     const float sigma = g0.w;
     const float F = dL[0] * col.x + dL[1] * col.y + dL[2] * col.z;
     float Ss = 0.0f, S1 = 0.0f, S2 = 0.0f, Sc = 0.0f;
@@ -161,7 +200,7 @@ __device__ __forceinline__ void grad_hit(const Isect& I, int slot, float4 g0, fl
         if (SPB > 8 && i >= p.spb) break;
         const float t = sample_t(tb, i, p.dt);
         const float tau = t - I.tc;
-        const float pw = I.A + I.Cq * (tau * tau);
+        const float pw = fmaf(I.Cq, tau * tau, I.A);  // explicit: same rounding in every pass
         const float w = __expf(-pw);
         if (sigma * w > p.sigma_thr) {
             const float phi = Aq[i] + Bq[i] * (F - Eq[i]);
@@ -177,19 +216,20 @@ __device__ __forceinline__ void grad_hit(const Isect& I, int slot, float4 g0, fl
     const float S0 = sigma * Ss, T1 = sigma * S1, T2 = sigma * S2;
     // y(t) = yc + tau v in the unenlarged ellipsoid frame
     const float yc[3] = {d * I.Px, d * I.Py, d * I.Pz};
-    const float v[3] = {d * I.Dx, d * I.Dy, d * I.Dz};
+    const float vv[3] = {d * I.Dx, d * I.Dy, d * I.Dz};
     float Gy[3], Y[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        Gy[a] = yc[a] * S0 + v[a] * T1;
+        Gy[a] = yc[a] * S0 + vv[a] * T1;
 #pragma unroll
-        for (int b = 0; b < 3; ++b) Y[a][b] = yc[a] * yc[b] * S0 + (yc[a] * v[b] + v[a] * yc[b]) * T1 + v[a] * v[b] * T2;
+        for (int b = 0; b < 3; ++b)
+            Y[a][b] = yc[a] * yc[b] * S0 + (yc[a] * vv[b] + vv[a] * yc[b]) * T1 + vv[a] * vv[b] * T2;
     }
     const float m[3][3] = {{g1.x, g1.y, g1.z}, {g2.x, g2.y, g2.z}, {g3.x, g3.y, g3.z}};  // rows of M'
     const float s[3] = {b0.x, b0.y, b0.z};
-    float dmu[3], dcol[3][3];
+    float dcol[3][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dmu[c] = d * (m[0][c] * Gy[0] + m[1][c] * Gy[1] + m[2][c] * Gy[2]);
+    for (int c = 0; c < 3; ++c) v[c] = d * (m[0][c] * Gy[0] + m[1][c] * Gy[1] + m[2][c] * Gy[2]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float f = -d / s[k];
@@ -203,38 +243,50 @@ __device__ __forceinline__ void grad_hit(const Isect& I, int slot, float4 g0, fl
     const float* c0 = dcol[0];
     const float* c1 = dcol[1];
     const float* c2 = dcol[2];
-    const float dqr = 2.0f * (qk * c0[1] - qj * c0[2] - qk * c1[0] + qi * c1[2] + qj * c2[0] - qi * c2[1]);
-    const float dqi = 2.0f * (qj * c0[1] + qk * c0[2] + qj * c1[0] + r * c1[2] + qk * c2[0] - r * c2[1]) -
-                      4.0f * qi * (c1[1] + c2[2]);
-    const float dqj = 2.0f * (qi * c0[1] - r * c0[2] + qi * c1[0] + qk * c1[2] + r * c2[0] + qk * c2[1]) -
-                      4.0f * qj * (c0[0] + c2[2]);
-    const float dqk = 2.0f * (r * c0[1] + qi * c0[2] - r * c1[0] + qj * c1[2] + qi * c2[0] + qj * c2[1]) -
-                      4.0f * qk * (c0[0] + c1[1]);
+    v[3] = Ss;
+    v[4] = Y[0][0] / s[0];
+    v[5] = Y[1][1] / s[1];
+    v[6] = Y[2][2] / s[2];
+    v[7] = 2.0f * (qk * c0[1] - qj * c0[2] - qk * c1[0] + qi * c1[2] + qj * c2[0] - qi * c2[1]);
+    v[8] = 2.0f * (qj * c0[1] + qk * c0[2] + qj * c1[0] + r * c1[2] + qk * c2[0] - r * c2[1]) -
+           4.0f * qi * (c1[1] + c2[2]);
+    v[9] = 2.0f * (qi * c0[1] - r * c0[2] + qi * c1[0] + qk * c1[2] + r * c2[0] + qk * c2[1]) -
+           4.0f * qj * (c0[0] + c2[2]);
+    v[10] = 2.0f * (r * c0[1] + qi * c0[2] - r * c1[0] + qj * c1[2] + qi * c2[0] + qj * c2[1]) -
+            4.0f * qk * (c0[0] + c1[1]);
+    const float sc_ = sigma * Sc;
+    v[11] = dL[0] * sc_;
+    v[12] = dL[1] * sc_;
+    v[13] = dL[2] * sc_;
+}
+
+// 14 FP32 atomic adds of one Gaussian's (summed) gradient contribution.
+__device__ __forceinline__ void grad_scatter(const float* v, int slot, int view, const DevScene& sc,
+                                             const DevGrad& gr) {
     const bool local = gr.copies > 1;
     const int copy = local ? xcc_id() : 0;
     float* acc = gr.acc + (size_t)copy * gr.acc_stride + (size_t)kAccStride * slot;
-    gadd(acc + 0, dmu[0], local);
-    gadd(acc + 1, dmu[1], local);
-    gadd(acc + 2, dmu[2], local);
-    gadd(acc + 3, Ss, local);
-    gadd(acc + 4, Y[0][0] / s[0], local);
-    gadd(acc + 5, Y[1][1] / s[1], local);
-    gadd(acc + 6, Y[2][2] / s[2], local);
-    gadd(acc + 8, dqr, local);
-    gadd(acc + 9, dqi, local);
-    gadd(acc + 10, dqj, local);
-    gadd(acc + 11, dqk, local);
+    gadd(acc + 0, v[0], local);
+    gadd(acc + 1, v[1], local);
+    gadd(acc + 2, v[2], local);
+    gadd(acc + 3, v[3], local);
+    gadd(acc + 4, v[4], local);
+    gadd(acc + 5, v[5], local);
+    gadd(acc + 6, v[6], local);
+    gadd(acc + 8, v[7], local);
+    gadd(acc + 9, v[8], local);
+    gadd(acc + 10, v[9], local);
+    gadd(acc + 11, v[10], local);
     float* dr = reinterpret_cast<float*>(gr.drgb + (size_t)copy * gr.drgb_stride + (size_t)view * sc.n + slot);
-    const float sc_ = sigma * Sc;
-    gadd(dr + 0, dL[0] * sc_, local);
-    gadd(dr + 1, dL[1] * sc_, local);
-    gadd(dr + 2, dL[2] * sc_, local);
+    gadd(dr + 0, v[11], local);
+    gadd(dr + 1, v[12], local);
+    gadd(dr + 2, v[13], local);
 }
 
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }
 
 // Gathers the intervals overlapping [ta, tb] into the lane's list; returns the count, or
-// CAP + 1 on overflow.
+// CAP + 1 on overflow.  Entries come out in DFS order of the tree = increasing Morton slot.
 template <int CAP>
 __device__ __forceinline__ int gather(const DevScene& sc, const Ray& ray, float ta, float tb, IList<CAP>& L,
                                       int lane) {
@@ -249,57 +301,97 @@ __device__ __forceinline__ int gather(const DevScene& sc, const Ray& ray, float 
             cnt = CAP + 1;
             return false;
         }
+        refine_sampling(g0, g1, g2, g3, ray, I);
         L.g[cnt][lane] = slot;
         L.r1[cnt][lane] = I.r1;
         L.r2[cnt][lane] = I.r2;
+        L.tc[cnt][lane] = I.tc;
+        L.A[cnt][lane] = I.A;
+        L.Cq[cnt][lane] = I.Cq;
         ++cnt;
         return true;
     });
     return cnt;
 }
 
-// Shared segment driver.  Calls slab_list(tb, tmin, tmax, cnt) for slabs covered by the list
-// and slab_direct(tb, tmin, tmax) for slabs whose hit set exceeds the list.  Returns when the ray
-// leaves the scene or `alive()` turns false.
-template <int CAP, typename FL, typename FD, typename AL>
-__device__ __forceinline__ void march(const DevScene& sc, const DevParams& p, const Ray& ray, IList<CAP>& L, int lane,
-                                      Counters& ctr, FL&& slab_list, FD&& slab_direct, AL&& alive) {
+// ------------------------------------------------------------------ slab stepper -------------
+//
+// The reference's slab loop, with hit sets served from gathered segments.  next_slab() returns
+// the next slab to process (tb, tmin, tmax) in either list mode (s.cnt entries in the lane's
+// list) or direct mode (the slab's hit set overflows the list: per-slab traversal); empty
+// segments are skipped without returning.  advance_slab() moves past the slab just processed.
+
+struct SegState {
+    float tb;       // start of the current slab (reference tbuffer)
+    float seg_hi;   // end of the gathered segment
+    int m;          // segment length in slabs for the next gather
+    int cnt;        // entries in the lane's list
+    bool valid;     // list holds the current segment
+    bool direct;    // current slab is processed by per-slab traversal
+};
+
+__device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevParams& p) {
+    s.tb = ray.tenter;
+    s.seg_hi = ray.tenter;
+    s.m = p.seg_slabs;
+    s.cnt = 0;
+    s.valid = false;
+    s.direct = false;
+}
+
+template <int CAP>
+__device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const DevParams& p, const Ray& ray,
+                                          IList<CAP>& L, int lane, Counters& ctr, float& tmin, float& tmax) {
     const float slab = p.dt * (float)p.spb;
-    float tb = ray.tenter;
-    int m = p.seg_slabs;
-    if (!(add_exact(tb, slab) > tb)) return;  // slab below half an ulp of t: no progress possible
-    while (tb < ray.texit && alive()) {
-        const float seg_hi = fminf(ray.texit, tb + (float)m * slab);
-        const int cnt = gather<CAP>(sc, ray, tb, seg_hi, L, lane);
-        ctr.gathers++;
-        if (cnt > CAP) {
-            if (m > 1) {
-                m = m >> 1;
+    while (true) {
+        if (!(s.tb < ray.texit)) return false;
+        if (!s.valid) {
+            s.seg_hi = fminf(ray.texit, s.tb + (float)s.m * slab);
+            const int cnt = gather<CAP>(sc, ray, s.tb, s.seg_hi, L, lane);
+            ctr.gathers++;
+            if (cnt > CAP) {
+                if (s.m > 1) {
+                    s.m >>= 1;
+                    continue;
+                }
+                ctr.fallback++;
+                s.direct = true;
+                s.cnt = 0;
+            } else {
+                s.direct = false;
+                s.cnt = cnt;
+            }
+            s.valid = true;
+        }
+        tmax = fminf(ray.texit, add_exact(s.tb, slab));
+        if (!s.direct) {
+            if (tmax > s.seg_hi) {  // segment done: adapt its length to the list fill, gather the next
+                if (s.cnt == 0 || 4 * s.cnt < CAP) s.m = min(s.m * 2, kMaxSeg);
+                else if (4 * s.cnt > 3 * CAP) s.m = max(s.m >> 1, 1);
+                s.valid = false;
                 continue;
             }
-            // one slab with more hits than the list holds: per-slab traversal
-            ctr.fallback++;
-            const float tmin = fmaxf(ray.tenter, tb);
-            const float tmax = fminf(ray.texit, add_exact(tb, slab));
-            if (tmax > ray.tenter) slab_direct(tb, tmin, tmax);
-            const float nt = add_exact(tb, slab);
-            if (!(nt > tb)) return;
-            tb = nt;
-            continue;
+            if (s.cnt == 0) {  // empty segment: its slabs have no hits
+                const float nt = add_exact(s.tb, slab);
+                if (!(nt > s.tb)) return false;
+                s.tb = nt;
+                continue;
+            }
         }
-        // slabs fully inside the gathered segment
-        while (tb < ray.texit && alive()) {
-            const float tmax = fminf(ray.texit, add_exact(tb, slab));
-            if (tmax > seg_hi) break;
-            const float tmin = fmaxf(ray.tenter, tb);
-            if (cnt > 0 && tmax > ray.tenter) slab_list(tb, tmin, tmax, cnt);
-            const float nt = add_exact(tb, slab);
-            if (!(nt > tb)) return;
-            tb = nt;
-        }
-        if (cnt == 0 || 4 * cnt < CAP) m = min(m * 2, kMaxSeg);
-        else if (4 * cnt > 3 * CAP) m = max(m >> 1, 1);
+        tmin = fmaxf(ray.tenter, s.tb);
+        return true;
     }
+}
+
+__device__ __forceinline__ bool advance_slab(SegState& s, const DevParams& p) {
+    const float nt = add_exact(s.tb, p.dt * (float)p.spb);
+    if (s.direct) {
+        s.direct = false;
+        s.valid = false;
+    }
+    if (!(nt > s.tb)) return false;  // slab below half an ulp of t: no progress possible
+    s.tb = nt;
+    return true;
 }
 
 struct RayRef {
@@ -309,8 +401,7 @@ struct RayRef {
     bool active;
 };
 
-// Maps a work index to (view, pixel, output row).  Single-view implicit mode walks the image in
-// 8x8 pixel tiles so a wavefront's rays are spatially coherent.
+// Maps a work index to (view, pixel, output row).
 __device__ __forceinline__ RayRef ray_ref(const RayJob& job, int64_t r) {
     RayRef q;
     q.active = false;
@@ -371,126 +462,183 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
     o.depth = 0.0f;
     o.T = 1.0f;
     if (!(ray.tenter < ray.texit)) return;
-    float first_tb = ray.tenter;
-    auto eval_hit = [&](int slot, const Isect& I, float tb, float tmin, float* sig, float* cr, float* cg, float* cb) {
-        const float4 g0 = sc.geo[4 * slot];
-        const float4 col = rgbv[slot];
-        accum_hit<SPB>(I, g0.w, col, tb, p, sig, cr, cg, cb);
-        if (tb == first_tb || I.r1 > tmin) ctr.P++;
-    };
-    auto finish_slab = [&](int nh, float tb, const float* sig, const float* cr, const float* cg, const float* cb) {
+    if (!(add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter)) return;
+    SegState s;
+    seg_init(s, ray, p);
+    const float first_tb = ray.tenter;
+    while (o.T > p.t_eps) {
+        float tmin, tmax;
+        if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) break;
+        const float tb = s.tb;
+        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
+#pragma unroll
+        for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
+        int nh = 0;
+        auto eval_hit = [&](int slot, const Isect& I, float sigma) {
+            accum_hit<SPB>(I, sigma, rgbv[slot], tb, p, sig, cr, cg, cb);
+            if (tb == first_tb || I.r1 > tmin) ctr.P++;
+            ++nh;
+        };
+        if (s.direct) {
+            traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+                float4 g0, g1, g2, g3;
+                load_rec(sc.geo, slot, g0, g1, g2, g3);
+                Isect I;
+                if (!intersect(g0, g1, g2, g3, ray, I)) return true;
+                if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+                refine_sampling(g0, g1, g2, g3, ray, I);
+                eval_hit(slot, I, g0.w);
+                return true;
+            });
+        } else {
+            for (int e = 0; e < s.cnt; ++e) {
+                if (!overlaps(tmin, tmax, L.r1[e][lane], L.r2[e][lane])) continue;
+                const int slot = L.g[e][lane];
+                eval_hit(slot, entry_isect(L, e, lane), sc.geo[4 * slot].w);
+            }
+        }
         ctr.K += nh;
         if (p.max_hits > 0 && nh > p.max_hits) ctr.ovf++;
         if (nh > 0) composite<SPB>(sig, cr, cg, cb, tb, p, o);
-    };
-    auto slab_list = [&](float tb, float tmin, float tmax, int cnt) {
-        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
-#pragma unroll
-        for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
-        int nh = 0;
-        for (int e = 0; e < cnt; ++e) {
-            const float r1 = L.r1[e][lane], r2 = L.r2[e][lane];
-            if (!overlaps(tmin, tmax, r1, r2)) continue;
-            const int slot = L.g[e][lane];
-            float4 g0, g1, g2, g3;
-            load_rec(sc.geo, slot, g0, g1, g2, g3);
-            Isect I;
-            intersect(g0, g1, g2, g3, ray, I);
-            refine_sampling(g0, g1, g2, g3, ray, I);
-            eval_hit(slot, I, tb, tmin, sig, cr, cg, cb);
-            ++nh;
-        }
-        finish_slab(nh, tb, sig, cr, cg, cb);
-    };
-    auto slab_direct = [&](float tb, float tmin, float tmax) {
-        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
-#pragma unroll
-        for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
-        int nh = 0;
-        traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
-            float4 g0, g1, g2, g3;
-            load_rec(sc.geo, slot, g0, g1, g2, g3);
-            Isect I;
-            if (!intersect(g0, g1, g2, g3, ray, I)) return true;
-            if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
-            refine_sampling(g0, g1, g2, g3, ray, I);
-            eval_hit(slot, I, tb, tmin, sig, cr, cg, cb);
-            ++nh;
-            return true;
-        });
-        finish_slab(nh, tb, sig, cr, cg, cb);
-    };
-    march<CAP>(sc, p, ray, L, lane, ctr, slab_list, slab_direct, [&]() { return o.T > p.t_eps; });
+        if (!advance_slab(s, p)) break;
+    }
 }
 
+// Backward of one ray per lane, called by all 64 lanes of the wave together (inactive lanes pass
+// active = false).  Each outer iteration every live lane prepares its next slab that has hits
+// (recomputed buffers, per-sample coefficients, T and Crem advanced); then the wave walks the
+// union of the prepared slabs' hits in increasing (view, slot) order and scatters one summed
+// contribution per Gaussian.  Slabs served by per-slab traversal scatter per lane.
 template <int SPB, int CAP>
-__device__ void backward_ray(const DevScene& sc, const DevParams& p, const Ray& ray, const float4* __restrict__ rgbv,
-                             const float* Cfwd, const float* dL, IList<CAP>& L, int lane, Counters& ctr,
-                             const DevGrad& gr, int view) {
-    if (!(ray.tenter < ray.texit)) return;
+__device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray,
+                              const float4* __restrict__ rgbv, const float* Cfwd, const float* dL, IList<CAP>& L,
+                              int lane, const DevGrad& gr, int view, Counters& ctr) {
+    bool alive = active && (ray.tenter < ray.texit) && (add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter);
     float T = 1.0f;
     float Crem[3] = {Cfwd[0], Cfwd[1], Cfwd[2]};
-    auto slab_list = [&](float tb, float tmin, float tmax, int cnt) {
-        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
+    SegState s;
+    seg_init(s, ray, p);
+    const int key_base = view * sc.n;
+    const float window = p.dt * (float)p.spb;  // lanes within one slab of the laggard step together
+    float Aq[SPB], Bq[SPB], Eq[SPB];
 #pragma unroll
-        for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
-        int nh = 0;
-        for (int e = 0; e < cnt; ++e) {
-            const float r1 = L.r1[e][lane], r2 = L.r2[e][lane];
-            if (!overlaps(tmin, tmax, r1, r2)) continue;
-            const int slot = L.g[e][lane];
-            float4 g0, g1, g2, g3;
-            load_rec(sc.geo, slot, g0, g1, g2, g3);
-            Isect I;
-            intersect(g0, g1, g2, g3, ray, I);
-            refine_sampling(g0, g1, g2, g3, ray, I);
-            accum_hit<SPB>(I, g0.w, rgbv[slot], tb, p, sig, cr, cg, cb);
-            ++nh;
+    for (int i = 0; i < SPB; ++i) Aq[i] = Bq[i] = Eq[i] = 0.0f;
+    bool pend = false;  // next_slab() has produced a slab not yet processed
+    float tmin = 0.0f, tmax = 0.0f;
+    while (__ballot(alive) != 0ULL) {
+        bool has = false;
+        float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
+        int cur_cnt = 0;
+        if (alive && !pend) {
+            if (!(T > p.t_eps)) alive = false;
+            else if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) alive = false;
+            else pend = true;
         }
-        if (nh == 0) return;
-        slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
-        for (int e = 0; e < cnt; ++e) {
-            const float r1 = L.r1[e][lane], r2 = L.r2[e][lane];
-            if (!overlaps(tmin, tmax, r1, r2)) continue;
-            const int slot = L.g[e][lane];
-            float4 g0, g1, g2, g3;
-            load_rec(sc.geo, slot, g0, g1, g2, g3);
-            Isect I;
-            intersect(g0, g1, g2, g3, ray, I);
-            refine_sampling(g0, g1, g2, g3, ray, I);
-            grad_hit<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], tb, p, sig, cr, cg, dL, sc, gr, view);
-        }
-    };
-    auto slab_direct = [&](float tb, float tmin, float tmax) {
-        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
+        // depth synchronisation: the wave's merged scatter only pays off when the lanes' current
+        // slabs are at similar depth (similar Gaussians), so lanes ahead of the laggard wait
+        const float tsync = wave_min64f(alive && pend ? s.tb : INFINITY);
+        if (alive && pend && s.tb <= tsync + window) {
+            pend = false;
+            {
+                const float tb = s.tb;
+                float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
 #pragma unroll
-        for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
-        int nh = 0;
-        traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
-            float4 g0, g1, g2, g3;
-            load_rec(sc.geo, slot, g0, g1, g2, g3);
-            Isect I;
-            if (!intersect(g0, g1, g2, g3, ray, I)) return true;
-            if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
-            refine_sampling(g0, g1, g2, g3, ray, I);
-            accum_hit<SPB>(I, g0.w, rgbv[slot], tb, p, sig, cr, cg, cb);
-            ++nh;
-            return true;
-        });
-        if (nh == 0) return;
-        slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
-        traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
-            float4 g0, g1, g2, g3;
-            load_rec(sc.geo, slot, g0, g1, g2, g3);
-            Isect I;
-            if (!intersect(g0, g1, g2, g3, ray, I)) return true;
-            if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
-            refine_sampling(g0, g1, g2, g3, ray, I);
-            grad_hit<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], tb, p, sig, cr, cg, dL, sc, gr, view);
-            return true;
-        });
-    };
-    march<CAP>(sc, p, ray, L, lane, ctr, slab_list, slab_direct, [&]() { return T > p.t_eps; });
+                for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
+                int nh = 0;
+                if (s.direct) {
+                    traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+                        float4 g0, g1, g2, g3;
+                        load_rec(sc.geo, slot, g0, g1, g2, g3);
+                        Isect I;
+                        if (!intersect(g0, g1, g2, g3, ray, I)) return true;
+                        if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+                        refine_sampling(g0, g1, g2, g3, ray, I);
+                        accum_hit<SPB>(I, g0.w, rgbv[slot], tb, p, sig, cr, cg, cb);
+                        ++nh;
+                        return true;
+                    });
+                } else {
+                    for (int e = 0; e < s.cnt; ++e) {
+                        if (!overlaps(tmin, tmax, L.r1[e][lane], L.r2[e][lane])) continue;
+                        const int slot = L.g[e][lane];
+                        accum_hit<SPB>(entry_isect(L, e, lane), sc.geo[4 * slot].w, rgbv[slot], tb, p, sig, cr, cg,
+                                       cb);
+                        ++nh;
+                    }
+                }
+                if (nh > 0) {
+                    slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
+                    if (s.direct) {
+                        // per-slab traversal: scatter per lane
+                        traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+                            float4 g0, g1, g2, g3;
+                            load_rec(sc.geo, slot, g0, g1, g2, g3);
+                            Isect I;
+                            if (!intersect(g0, g1, g2, g3, ray, I)) return true;
+                            if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+                            refine_sampling(g0, g1, g2, g3, ray, I);
+                            float v[kGradVals];
+                            grad_vals<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], tb, p, sig, cr, cg, dL, sc, v);
+                            grad_scatter(v, slot, view, sc, gr);
+                            return true;
+                        });
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < SPB; ++i) {
+                            Aq[i] = sig[i];
+                            Bq[i] = cr[i];
+                            Eq[i] = cg[i];
+                        }
+                        has = true;
+                        cur_tb = tb;
+                        cur_tmin = tmin;
+                        cur_tmax = tmax;
+                        cur_cnt = s.cnt;
+                    }
+                }
+                if (!advance_slab(s, p)) alive = false;
+            }
+        }
+        // converged: wave-merged scatter over the prepared slabs' hits
+        int e = 0;
+        while (true) {
+            int my = INT_MAX;
+            if (has) {
+                while (e < cur_cnt) {
+                    if (overlaps(cur_tmin, cur_tmax, L.r1[e][lane], L.r2[e][lane])) {
+                        my = key_base + L.g[e][lane];
+                        break;
+                    }
+                    ++e;
+                }
+            }
+            const int mn = wave_min64(my);
+            if (mn == INT_MAX) break;
+            const bool act = (my == mn);
+            float v[kGradVals];
+#pragma unroll
+            for (int k = 0; k < kGradVals; ++k) v[k] = 0.0f;
+            if (act) {
+                const int slot = L.g[e][lane];
+                float4 g0, g1, g2, g3;
+                load_rec(sc.geo, slot, g0, g1, g2, g3);
+                Isect I;
+                intersect(g0, g1, g2, g3, ray, I);
+                refine_sampling(g0, g1, g2, g3, ray, I);
+                grad_vals<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, p, Aq, Bq, Eq, dL, sc, v);
+                ++e;
+            }
+            const unsigned long long bal = __ballot(act);
+            const int mview = mn / sc.n, mslot = mn - mview * sc.n;
+            if (__popcll(bal) == 1) {
+                if (act) grad_scatter(v, mslot, mview, sc, gr);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kGradVals; ++k) v[k] = wave_sum64(v[k]);
+                if (lane == __ffsll((long long)bal) - 1) grad_scatter(v, mslot, mview, sc, gr);
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ kernels ------------------
@@ -516,8 +664,7 @@ __device__ __forceinline__ bool resolve(const RayJob& job, int64_t w, RayRef& q)
 __device__ __forceinline__ int64_t block_item(int64_t b, int64_t nblocks) {
     const int64_t per = (nblocks + 7) >> 3;
     const int64_t xcd = b & 7, k = b >> 3;
-    const int64_t logical = xcd * per + k;
-    return logical;
+    return xcd * per + k;
 }
 
 template <int SPB, int CAP>
@@ -554,27 +701,37 @@ __global__ void __launch_bounds__(kWave) k_backward(DevScene sc, DevParams p, Ra
     const int64_t item = block_item(blockIdx.x, nb) * kWave + lane;
     Counters ctr;
     RayRef q;
+    bool active = false;
+    Ray ray = {};
+    float dL[3] = {0.0f, 0.0f, 0.0f}, Cf[3] = {0.0f, 0.0f, 0.0f};
+    int view = 0;
     if (item < num_items && resolve(job, item, q)) {
-        const float dL[3] = {dLdC[3 * q.out], dLdC[3 * q.out + 1], dLdC[3 * q.out + 2]};
+        dL[0] = dLdC[3 * q.out];
+        dL[1] = dLdC[3 * q.out + 1];
+        dL[2] = dLdC[3 * q.out + 2];
         if (dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f) {
-            const float Cf[3] = {ray_colors[3 * q.out], ray_colors[3 * q.out + 1], ray_colors[3 * q.out + 2]};
+            active = true;
+            view = q.view;
+            Cf[0] = ray_colors[3 * q.out];
+            Cf[1] = ray_colors[3 * q.out + 1];
+            Cf[2] = ray_colors[3 * q.out + 2];
             const rg_camera cam = job.cams[q.view];
-            const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
-            backward_ray<SPB, CAP>(sc, p, ray, sc.rgb + (size_t)q.view * sc.n, Cf, dL, L, lane, ctr, gr, q.view);
+            ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
         }
     }
+    backward_wave<SPB, CAP>(sc, p, active, ray, sc.rgb + (size_t)view * sc.n, Cf, dL, L, lane, gr, view, ctr);
     flush_counters(ctr, st);
 }
 
-// Fused train step, persistent: waves pull chunks of `chunk` rays from a device queue; each ray
-// is rendered, its per-pixel loss gradient formed, and back-propagated in the same lane.
+// Fused train step, persistent: waves pull chunks of `chunk` rays from device queues; each ray
+// is rendered, its per-pixel loss gradient formed, and back-propagated by the same wave.
 template <int SPB, int CAP>
 __global__ void __launch_bounds__(kWave) k_train(DevScene sc, DevParams p, RayJob job, int64_t num_items, int chunk,
                                                  const float* __restrict__ target, int loss_kind,
                                                  float* __restrict__ out_rgb, DevGrad gr, DevStats st) {
     __shared__ IList<CAP> L;
     const int lane = threadIdx.x;
-    Counters ctr;
+    Counters ctr, dummy;
     float loss = 0.0f;
     // The work items are split into 8 contiguous ranges with one queue each; a wave drains the
     // queue of the XCD it runs on first (that XCD's L2 then serves a compact part of the scene)
@@ -582,51 +739,55 @@ __global__ void __launch_bounds__(kWave) k_train(DevScene sc, DevParams p, RayJo
     const int xcc = xcc_id();
     const int64_t per = (num_items + 7) / 8;
     for (int k = 0; k < 8; ++k) {
-      const int qid = (xcc + k) & 7;
-      const int64_t q0 = (int64_t)qid * per;
-      const int64_t q1 = min(num_items, q0 + per);
-      while (q0 < q1) {
-        int64_t base = 0;
-        if (lane == 0) base = (int64_t)atomicAdd(&st.c[C_QUEUE0 + qid], (unsigned long long)chunk);
-        base = __shfl(base, 0, 64);
-        if (q0 + base >= q1) break;
-        for (int j = 0; j < chunk; j += kWave) {
-            const int64_t item = q0 + base + j + lane;
-            RayRef q;
-            if (item < q1 && resolve(job, item, q)) {
-                const rg_camera cam = job.cams[q.view];
-                const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
-                const float4* rgbv = sc.rgb + (size_t)q.view * sc.n;
-                RayOut o;
-                forward_ray<SPB, CAP>(sc, p, ray, rgbv, L, lane, ctr, o);
-                float C[3] = {o.C[0] + o.T * p.bg, o.C[1] + o.T * p.bg, o.C[2] + o.T * p.bg};
-                if (out_rgb) {
-                    out_rgb[3 * q.out + 0] = C[0];
-                    out_rgb[3 * q.out + 1] = C[1];
-                    out_rgb[3 * q.out + 2] = C[2];
-                }
-                float dL[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float e = C[k] - target[3 * q.out + k];
-                    if (loss_kind == 1) {
-                        loss += fabsf(e);
-                        dL[k] = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
-                    } else {
-                        loss += e * e;
-                        dL[k] = 2.0f * e;
+        const int qid = (xcc + k) & 7;
+        const int64_t q0 = (int64_t)qid * per;
+        const int64_t q1 = min(num_items, q0 + per);
+        while (q0 < q1) {
+            int64_t base = 0;
+            if (lane == 0) base = (int64_t)atomicAdd(&st.c[C_QUEUE0 + qid], (unsigned long long)chunk);
+            base = __shfl(base, 0, 64);
+            if (q0 + base >= q1) break;
+            for (int j = 0; j < chunk; j += kWave) {
+                const int64_t item = q0 + base + j + lane;
+                RayRef q;
+                bool act = false;
+                Ray ray = {};
+                float C[3] = {0.0f, 0.0f, 0.0f}, dL[3] = {0.0f, 0.0f, 0.0f};
+                int view = 0;
+                if (item < q1 && resolve(job, item, q)) {
+                    const rg_camera cam = job.cams[q.view];
+                    ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
+                    view = q.view;
+                    RayOut o;
+                    forward_ray<SPB, CAP>(sc, p, ray, sc.rgb + (size_t)view * sc.n, L, lane, ctr, o);
+                    C[0] = o.C[0] + o.T * p.bg;
+                    C[1] = o.C[1] + o.T * p.bg;
+                    C[2] = o.C[2] + o.T * p.bg;
+                    if (out_rgb) {
+                        out_rgb[3 * q.out + 0] = C[0];
+                        out_rgb[3 * q.out + 1] = C[1];
+                        out_rgb[3 * q.out + 2] = C[2];
                     }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const float e = C[c] - target[3 * q.out + c];
+                        if (loss_kind == 1) {
+                            loss += fabsf(e);
+                            dL[c] = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
+                        } else {
+                            loss += e * e;
+                            dL[c] = 2.0f * e;
+                        }
+                    }
+                    act = dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f;
                 }
-                if (dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f) {
-                    Counters dummy;
-                    backward_ray<SPB, CAP>(sc, p, ray, rgbv, C, dL, L, lane, dummy, gr, q.view);
-                }
+                backward_wave<SPB, CAP>(sc, p, act, ray, sc.rgb + (size_t)view * sc.n, C, dL, L, lane, gr, view,
+                                        dummy);
             }
         }
-      }
     }
     flush_counters(ctr, st);
-    const float ls = wave_sum(loss);
+    const float ls = wave_sum64(loss);
     if (lane == 0 && ls != 0.0f) atomicAdd(reinterpret_cast<float*>(&st.c[C_LOSS]), ls);
 }
 
@@ -669,7 +830,9 @@ __global__ void __launch_bounds__(kWave) k_debug_hits(DevScene sc, DevParams p, 
                     return true;
                 });
             }
-            tb = add_exact(tb, slab);
+            const float nt = add_exact(tb, slab);
+            if (!(nt > tb)) break;
+            tb = nt;
         }
     }
     counts[r] = n;
@@ -678,8 +841,6 @@ __global__ void __launch_bounds__(kWave) k_debug_hits(DevScene sc, DevParams p, 
 __global__ void k_reset_counters(unsigned long long* c) {
     if (threadIdx.x < 16) c[threadIdx.x] = 0;
 }
-
-// ------------------------------------------------------------------ dispatch -----------------
 
 }  // namespace
 
