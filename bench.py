@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--workload", default="unbounded1m_8views")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-view", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="run the batched step eagerly instead of as a HIP graph")
     ap.add_argument("--hit-depth", type=int, default=16)
     ap.add_argument("--seg-slabs", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=128)
@@ -141,18 +142,7 @@ def main():
     prepare(rebuild=True)
     rc.set_views(cams)
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    kernel_ms = []
-
-    def step_batched(record=False):
-        prepare()
-        rc.set_views(cams)
-        if record:
-            ev[0].record()
-        rc.train_batch(ray_view, ray_pixel, target, 0, None, loss)
-        if record:
-            ev[1].record()
-        rc.finish_grads()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
 
     def step_per_view():
         prepare()
@@ -163,33 +153,31 @@ def main():
             rc.backward(v, px_dev[v], rgb, dL)
         rc.finish_grads()
 
-    def timed(fn, steps, warmup, record=False):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        kms = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn(record=True) if record else fn()
-            if record:
-                kms.append(ev)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / steps
-        return dt, kms
+    # ---- batched: refit + colour + fused persistent train kernel + SH/SG backward, one HIP graph ----
+    from ray_gauss_amd.batch import BatchedTrainer
 
-    # ---- batched (fused persistent train kernel) ----
+    rc_b = RayCaster(wl.params, tune)
+    trainer = BatchedTrainer(rc_b, t, cams, ray_view, ray_pixel, target, loss_kind=0, use_graph=not args.no_graph)
     for _ in range(args.warmup):
-        step_batched()
+        trainer.step()
     torch.cuda.synchronize()
-    per_kernel = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step_batched(record=True)
+        trainer.step()
+    torch.cuda.synchronize()
+    dt_b = (time.perf_counter() - t0) / args.steps
+    # the dominant kernel timed alone with HIP events on the stream it is launched on
+    per_kernel = []
+    for _ in range(2):
+        ev[0].record()
+        rc_b.train_batch(ray_view, ray_pixel, target, 0, None, loss)
+        ev[1].record()
         torch.cuda.synchronize()
         per_kernel.append(ev[0].elapsed_time(ev[1]))
-    dt_b = (time.perf_counter() - t0) / args.steps
-    st = rc.stats()
+    st = rc_b.stats()
+    rc_b.zero_grads()
     k_ms = float(np.mean(per_kernel))
+    del trainer
 
     # ---- per-view plain launches ----
     dt_pv = None

@@ -27,6 +27,7 @@ class TuneConfig:
     seg_slabs: int = 8
     chunk: int = 128
     xcd_grads: bool = True
+    packet: bool = True
 
 
 def _ptr(t):
@@ -105,6 +106,7 @@ class RayCaster:
         c.chunk = int(self.tune.chunk)
         c.max_hits_per_slab = int(self.max_hits_per_slab)
         c.xcd_local_grads = int(bool(self.tune.xcd_grads))
+        c.packet_traversal = int(bool(self.tune.packet))
         return c
 
     def set_tune(self, tune: TuneConfig):

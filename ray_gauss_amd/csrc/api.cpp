@@ -155,6 +155,7 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.bg = h->cfg.white_background ? 1.0f : 0.0f;
     p.max_hits = h->cfg.max_hits_per_slab;
     p.seg_slabs = h->cfg.seg_slabs;
+    p.packet = h->cfg.packet_traversal;
     return p;
 }
 

@@ -186,6 +186,40 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, int t
     }
 }
 
+// Packet form of traverse(): the lanes executing the call walk ONE node sequence together (the
+// node index is wave-uniform, so node loads are uniform), each lane testing the boxes against its
+// own ray segment; the wave descends into a node when any lane's segment meets its box, and at a
+// leaf every lane whose segment meets the leaf box calls on_leaf.  Each lane therefore sees
+// exactly the leaves the per-lane traversal would give it, in the same DFS order.  For the rays
+// of an 8x8 pixel tile the union of visited nodes is close to one ray's set.
+template <typename F>
+__device__ __forceinline__ void traverse_packet(const float4* __restrict__ nodes, int total, bool active, const Ray& r,
+                                                float ta, float tb, F&& on_leaf) {
+    const float ix = 1.0f / r.dx, iy = 1.0f / r.dy, iz = 1.0f / r.dz;
+    bool live = active;
+    int idx = 0;
+    while (idx < total && __ballot(live) != 0ULL) {
+        idx = __builtin_amdgcn_readfirstlane(idx);
+        const float4 lo = nodes[2 * idx];
+        const float4 hi = nodes[2 * idx + 1];
+        const int miss = __float_as_int(lo.w);
+        const int leaf = __float_as_int(hi.w);
+        if (miss <= idx) return;  // malformed link: never loop (idx strictly increases)
+        const bool h = live && box_hit(lo, hi, r, ix, iy, iz, ta, tb);
+        if (__ballot(h) != 0ULL) {
+            if (leaf >= 0) {
+                if (leaf >= ((total + 1) >> 1)) return;  // malformed leaf index: stop
+                if (h && !on_leaf(leaf)) live = false;
+                idx = miss;
+            } else {
+                idx = idx + 1;
+            }
+        } else {
+            idx = miss;
+        }
+    }
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;

@@ -47,6 +47,7 @@ struct DevParams {
     float bg;
     int max_hits;    // >0: count slabs with more hits (reference max_prim_slice)
     int seg_slabs;   // initial segment length
+    int packet;      // 1: wave-uniform packet traversal, 0: per-lane traversal
 };
 
 // Gradient accumulators.  With copies == 8 each XCD adds into its own copy with workgroup-scope

@@ -285,13 +285,22 @@ __device__ __forceinline__ void grad_scatter(const float* v, int slot, int view,
 
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }
 
+// LBVH traversal of the lanes executing the call: wave-uniform packet walk or per-lane walk.
+template <typename F>
+__device__ __forceinline__ void trav(const DevParams& p, const DevScene& sc, const Ray& ray, float ta, float tb,
+                                     F&& on_leaf) {
+    if (p.packet) traverse_packet(sc.nodes, sc.total_nodes, true, ray, ta, tb, on_leaf);
+    else traverse(sc.nodes, sc.total_nodes, ray, ta, tb, on_leaf);
+}
+
 // Gathers the intervals overlapping [ta, tb] into the lane's list; returns the count, or
 // CAP + 1 on overflow.  Entries come out in DFS order of the tree = increasing Morton slot.
 template <int CAP>
-__device__ __forceinline__ int gather(const DevScene& sc, const Ray& ray, float ta, float tb, IList<CAP>& L,
+__device__ __forceinline__ int gather(const DevScene& sc, const DevParams& p, const Ray& ray, float ta, float tb,
+                                      IList<CAP>& L,
                                       int lane) {
     int cnt = 0;
-    traverse(sc.nodes, sc.total_nodes, ray, ta, tb, [&](int slot) -> bool {
+    trav(p, sc, ray, ta, tb, [&](int slot) -> bool {
         float4 g0, g1, g2, g3;
         load_rec(sc.geo, slot, g0, g1, g2, g3);
         Isect I;
@@ -347,7 +356,7 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
         if (!(s.tb < ray.texit)) return false;
         if (!s.valid) {
             s.seg_hi = fminf(ray.texit, s.tb + (float)s.m * slab);
-            const int cnt = gather<CAP>(sc, ray, s.tb, s.seg_hi, L, lane);
+            const int cnt = gather<CAP>(sc, p, ray, s.tb, s.seg_hi, L, lane);
             ctr.gathers++;
             if (cnt > CAP) {
                 if (s.m > 1) {
@@ -480,7 +489,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
             ++nh;
         };
         if (s.direct) {
-            traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+            trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
                 float4 g0, g1, g2, g3;
                 load_rec(sc.geo, slot, g0, g1, g2, g3);
                 Isect I;
@@ -546,7 +555,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
                 int nh = 0;
                 if (s.direct) {
-                    traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+                    trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
                         float4 g0, g1, g2, g3;
                         load_rec(sc.geo, slot, g0, g1, g2, g3);
                         Isect I;
@@ -570,7 +579,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
                     if (s.direct) {
                         // per-slab traversal: scatter per lane
-                        traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+                        trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
                             float4 g0, g1, g2, g3;
                             load_rec(sc.geo, slot, g0, g1, g2, g3);
                             Isect I;
@@ -816,7 +825,7 @@ __global__ void __launch_bounds__(kWave) k_debug_hits(DevScene sc, DevParams p, 
             const float tmin = fmaxf(ray.tenter, tb);
             const float tmax = fminf(ray.texit, add_exact(tb, slab));
             if (tmax > ray.tenter) {
-                traverse(sc.nodes, sc.total_nodes, ray, tmin, tmax, [&](int slot) -> bool {
+                trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
                     float4 g0, g1, g2, g3;
                     load_rec(sc.geo, slot, g0, g1, g2, g3);
                     Isect I;
