@@ -6,8 +6,9 @@ against a fixed synthetic target), and the SH/SG backward.  Inputs are resident 
 the timed region.
 
 Two launch structures are timed on the same step:
-  batched  -- one persistent fused train kernel over all views (device work queue), the number
-              reported as `value`;
+  batched  -- one forward launch and one backward launch (which forms the loss gradient itself)
+              over the rays of all views, captured with the rest of the step in one HIP graph:
+              the number reported as `value`;
   per_view -- the plain path: one forward launch and one backward launch per view.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
@@ -41,11 +42,31 @@ def tile_order_pixels(w, h):
 
 
 def algorithmic_bytes(R, P):
-    """Per-ray and per-(ray, Gaussian) HBM bytes of a fused forward+backward launch: per ray the
-    colour written / re-read and dL/dC read (36 B); per distinct pair the forward read of the
-    Gaussian (mu, s, q, sigma, rgb: 56 B), the backward re-read (56 B) and the gradient write
-    (14 fp32 = 56 B)."""
+    """Per-ray and per-(ray, Gaussian) HBM bytes of forward + backward: per ray the colour
+    written / re-read and dL/dC read (36 B); per distinct pair the forward read of the Gaussian
+    (mu, s, q, sigma, rgb: 56 B), the backward re-read (56 B) and the gradient write (14 fp32 =
+    56 B)."""
     return 36.0 * R + 168.0 * P
+
+
+def backward_bytes(R, P):
+    """The backward launch's share of algorithmic_bytes: colour re-read + dL/dC read per ray
+    (24 B), Gaussian re-read + gradient write per pair (112 B)."""
+    return 24.0 * R + 112.0 * P
+
+
+def forward_bytes(R, P):
+    """The forward launch's share: colour write per ray (12 B), Gaussian read per pair (56 B)."""
+    return 12.0 * R + 56.0 * P
+
+
+def step_bytes(R, P, n, K, L, V):
+    """Whole-step algorithmic bytes: rays and pairs, preprocessing (geometry read, box write),
+    SH/SG forward (coefficients read once, V colour tables written) and SH/SG backward (V colour
+    gradients read, coefficients read, coefficient gradients and direction term written)."""
+    coef = 4.0 * (3 * K + 7 * L)
+    return (algorithmic_bytes(R, P) + n * (44.0 + 24.0) + n * (coef + 12.0 * V)
+            + n * (12.0 * V + coef + coef + 12.0))
 
 
 def cpu_baseline(wl, threads=None):
@@ -85,7 +106,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="run the batched step eagerly instead of as a HIP graph")
     ap.add_argument("--hit-depth", type=int, default=16)
     ap.add_argument("--seg-slabs", type=int, default=8)
-    ap.add_argument("--chunk", type=int, default=128)
+    ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -132,7 +153,7 @@ def main():
     targets_pv = [target[int(sum(len(p) for p in pixels[:v])):][:len(pixels[v])] for v in range(len(cams))]
     px_dev = [torch.from_numpy(p).to(dev) for p in pixels]
 
-    tune = TuneConfig(args.hit_depth, args.seg_slabs, args.chunk)
+    tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet)
     rc = RayCaster(wl.params, tune)
 
     def prepare(rebuild=False):
@@ -141,7 +162,6 @@ def main():
 
     prepare(rebuild=True)
     rc.set_views(cams)
-    loss = torch.zeros(1, dtype=torch.float32, device=dev)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
 
     def step_per_view():
@@ -153,7 +173,7 @@ def main():
             rc.backward(v, px_dev[v], rgb, dL)
         rc.finish_grads()
 
-    # ---- batched: refit + colour + fused persistent train kernel + SH/SG backward, one HIP graph ----
+    # ---- batched: refit + colour + forward + loss/backward + SH/SG backward, one HIP graph ----
     from ray_gauss_amd.batch import BatchedTrainer
 
     rc_b = RayCaster(wl.params, tune)
@@ -166,18 +186,26 @@ def main():
         trainer.step()
     torch.cuda.synchronize()
     dt_b = (time.perf_counter() - t0) / args.steps
-    # the dominant kernel timed alone with HIP events on the stream it is launched on
-    per_kernel = []
+    # the two ray kernels timed alone with HIP events on the stream they are launched on (the
+    # caster launches on torch's current stream)
+    f_ms, b_ms = [], []
     for _ in range(2):
         ev[0].record()
-        rc_b.train_batch(ray_view, ray_pixel, target, 0, None, loss)
+        rgb_b, _, _ = rc_b.forward_batch(ray_view, ray_pixel)
         ev[1].record()
         torch.cuda.synchronize()
-        per_kernel.append(ev[0].elapsed_time(ev[1]))
-    st = rc_b.stats()
-    rc_b.zero_grads()
-    k_ms = float(np.mean(per_kernel))
-    del trainer
+        f_ms.append(ev[0].elapsed_time(ev[1]))
+        st = rc_b.stats()
+        dL_b = 2.0 * (rgb_b - target)
+        torch.cuda.synchronize()
+        ev[0].record()
+        rc_b.backward_batch(ray_view, ray_pixel, rgb_b, dL_b)
+        ev[1].record()
+        torch.cuda.synchronize()
+        b_ms.append(ev[0].elapsed_time(ev[1]))
+        rc_b.zero_grads()
+    fk_ms, bk_ms = float(np.mean(f_ms)), float(np.mean(b_ms))
+    del trainer, rgb_b, dL_b
 
     # ---- per-view plain launches ----
     dt_pv = None
@@ -192,8 +220,9 @@ def main():
         dt_pv = (time.perf_counter() - t0) / args.steps
 
     P, K = st["pairs"], st["slab_hits"]
-    B = algorithmic_bytes(R, P)
-    achieved = B / (k_ms * 1e-3) / 1e9
+    B_bwd = backward_bytes(R, P)
+    achieved = B_bwd / (bk_ms * 1e-3) / 1e9
+    B_step = step_bytes(R, P, sc.n, (sc.sh_degree + 1) ** 2, L, len(cams))
     out = {
         "metric": "train rays/s (fwd+bwd) at 1M Gaussians, batched vs per-view; HBM GB/s",
         "value": R / dt_b,
@@ -210,15 +239,18 @@ def main():
         "config": {"workload": args.workload, "gaussians": sc.n, "views": len(cams),
                    "sizes": sorted({(c["width"], c["height"]) for c in cams})[:4], "rays_per_step": R,
                    "sh_degree": sc.sh_degree, "sg_lobes": L, "dt": wl.params.dt, "t_eps": wl.params.t_eps,
-                   "hit_depth": tune.hit_depth, "seg_slabs": tune.seg_slabs, "chunk": tune.chunk},
+                   "hit_depth": tune.hit_depth, "seg_slabs": tune.seg_slabs,
+                   "packet_traversal": tune.packet},
         "per_view_rays_per_s": (R / dt_pv) if dt_pv else None,
         "batched_over_per_view": (dt_pv / dt_b) if dt_pv else None,
         "counters": {"K_slab_hits": K, "P_pairs": P, "fallback_slabs": st["fallback_slabs"],
                      "gathers": st["gathers"]},
-        "roofline": {"bound": "hbm", "kernel": "k_train (fused forward+loss+backward)",
-                     "kernel_ms": k_ms, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "algorithmic_bytes": B},
+        "roofline": {"bound": "hbm", "kernel": "k_backward", "kernel_ms": bk_ms, "achieved": achieved,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes": B_bwd},
+        "forward_kernel": {"kernel": "k_forward", "kernel_ms": fk_ms,
+                           "achieved_GBps": forward_bytes(R, P) / (fk_ms * 1e-3) / 1e9},
+        "step_hbm_GBps": B_step / dt_b / 1e9,
     }
     if not args.no_cpu:
         try:

@@ -46,7 +46,6 @@ typedef struct rg_config {
     int32_t exact_sh_grad;     /* 0: backward_sh as the reference (utils/sh_utils.cu:254)        */
     int32_t hit_depth;         /* per-ray interval list capacity: 8, 16 or 32                    */
     int32_t seg_slabs;         /* initial slabs per interval gather: 1..4096                     */
-    int32_t chunk;             /* rays per work item in the persistent kernels: 64, 128, 256    */
     int32_t max_hits_per_slab; /* >0: count slabs whose hit set exceeds it (max_prim_slice)      */
     int32_t xcd_local_grads;   /* 1: one gradient copy per XCD (L2-local atomics), summed at the
                                   end; 0: one copy, device-scope atomics                          */
@@ -147,18 +146,18 @@ int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixe
 int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, const float* ray_colors,
                 const float* dL_dC, void* stream);
 
-/* Multi-view versions (persistent kernels pulling ray chunks from a device work queue). */
+/* Multi-view versions: ray r of the batch is pixel ray_pixel[r] of view ray_view[r]. */
 int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float* out_depth, float* out_opacity,
                      void* stream);
 int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colors, const float* dL_dC,
                       void* stream);
 
-/* Fused train step over a batch: forward, per-ray loss against target (loss_kind 0: L2 =
- * sum (C-target)^2, 1: L1 = sum |C-target|), backward -- in one persistent kernel, with no
- * intermediate colour buffer.  out_rgb (optional) receives the rendered colours, out_loss
- * (optional, device float[1]) the summed loss.  No reference counterpart: it fuses
- * RenderOptixFunction.forward + the caller's per-pixel loss + backward (scripts/train.py:104-114)
- * for per-pixel losses. */
+/* Train step over a batch: forward, per-ray loss against target (loss_kind 0: L2 =
+ * sum (C-target)^2, 1: L1 = sum |C-target|), backward -- two launches, the loss gradient formed
+ * inside the backward kernel (no dL/dC buffer).  out_rgb (optional) receives the rendered
+ * colours, out_loss (optional, device float[1]) the summed loss.  No reference counterpart: it
+ * fuses RenderOptixFunction.forward + the caller's per-pixel loss + backward
+ * (scripts/train.py:104-114) for per-pixel losses. */
 int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int32_t loss_kind, float* out_rgb,
                    float* out_loss, void* stream);
 

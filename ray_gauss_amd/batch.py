@@ -1,5 +1,5 @@
 """Multi-view batching: ray batches over views of uneven size, a hipGraph-captured train step
-(refit + colour + fused persistent forward/loss/backward + SH/SG backward), and the launch
+(refit + colour + forward + loss/backward + SH/SG backward), and the launch
 autotuner.
 
 The graph holds only kernel launches and device-to-device copies (no host transfers, no
@@ -60,9 +60,9 @@ def make_ray_batch(cams, pixels=None, device="cuda"):
 
 
 class BatchedTrainer:
-    """One training pass over a fixed ray batch: refit, colour, fused forward+loss+backward
-    (persistent work-queue kernel), SH/SG backward.  With use_graph=True the pass is captured in a
-    HIP graph on first call and replayed afterwards."""
+    """One training pass over a fixed ray batch: refit, colour, forward + loss + backward (a
+    forward launch, then a backward launch that forms the loss gradient), SH/SG backward.  With
+    use_graph=True the pass is captured in a HIP graph on first call and replayed afterwards."""
 
     def __init__(self, caster: RayCaster, params: dict, cams, ray_view, ray_pixel, target, loss_kind: int = 0,
                  use_graph: bool = True):
@@ -116,14 +116,14 @@ class BatchedTrainer:
 
 
 def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32), segs=(4, 8, 16),
-             chunks=(64, 128, 256), reps: int = 2, key: str | None = None, save: bool = True):
-    """Times the fused train kernel for every (hit_depth, seg_slabs, chunk) and keeps the fastest.
+             packets=(True, False), reps: int = 2, key: str | None = None, save: bool = True):
+    """Times the train step for every (hit_depth, seg_slabs, packet) and keeps the fastest.
     Returns (best TuneConfig, table of (config, ms))."""
     table = []
     loss = torch.zeros(1, dtype=torch.float32, device=ray_view.device)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for d, s, c in itertools.product(depths, segs, chunks):
-        cfg = TuneConfig(d, s, c)
+    for d, s, c in itertools.product(depths, segs, packets):
+        cfg = TuneConfig(d, s, packet=c)
         caster.set_tune(cfg)
         caster.train_batch(ray_view, ray_pixel, target, 0, None, loss)
         caster.zero_grads()
@@ -137,12 +137,12 @@ def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32),
         caster.zero_grads()
         table.append(((d, s, c), float(min(ms))))
     best = min(table, key=lambda r: r[1])[0]
-    cfg = TuneConfig(*best)
+    cfg = TuneConfig(best[0], best[1], packet=best[2])
     caster.set_tune(cfg)
     if save and key is not None:
         try:
             data = json.load(open(TUNE_FILE)) if os.path.exists(TUNE_FILE) else {}
-            data[key] = {"hit_depth": best[0], "seg_slabs": best[1], "chunk": best[2], "time": time.time()}
+            data[key] = {"hit_depth": best[0], "seg_slabs": best[1], "packet": bool(best[2]), "time": time.time()}
             os.makedirs(os.path.dirname(TUNE_FILE), exist_ok=True)
             json.dump(data, open(TUNE_FILE, "w"), indent=1)
         except OSError:
@@ -158,6 +158,6 @@ def tune_key(n: int, rays: int) -> str:
 def load_tuned(n: int, rays: int) -> TuneConfig | None:
     try:
         d = json.load(open(TUNE_FILE))[tune_key(n, rays)]
-        return TuneConfig(d["hit_depth"], d["seg_slabs"], d["chunk"])
+        return TuneConfig(d["hit_depth"], d["seg_slabs"], packet=bool(d["packet"]))
     except Exception:
         return None

@@ -20,12 +20,12 @@ from .cameras import launch_record_from
 
 @dataclass
 class TuneConfig:
-    """Launch tuning: interval-list depth per ray, initial slabs per gather, rays per queue item,
-    and whether gradients accumulate in one copy per XCD (L2-local atomics)."""
+    """Launch tuning: interval-list depth per ray, initial slabs per gather, whether gradients
+    accumulate in one copy per XCD (L2-local atomics), and whether a wavefront's lanes walk the
+    LBVH as one packet."""
 
-    hit_depth: int = 16
+    hit_depth: int = 8
     seg_slabs: int = 8
-    chunk: int = 128
     xcd_grads: bool = True
     packet: bool = True
 
@@ -103,7 +103,6 @@ class RayCaster:
         c.exact_sh_grad = int(self.exact_sh_grad)
         c.hit_depth = int(self.tune.hit_depth)
         c.seg_slabs = int(self.tune.seg_slabs)
-        c.chunk = int(self.tune.chunk)
         c.max_hits_per_slab = int(self.max_hits_per_slab)
         c.xcd_local_grads = int(bool(self.tune.xcd_grads))
         c.packet_traversal = int(bool(self.tune.packet))

@@ -36,6 +36,9 @@ struct rg_handle {
     float4* rgb = nullptr;
     float4* drgb = nullptr;
     int64_t rgb_cap = 0;  // floats4 allocated for rgb and drgb
+    // colour buffer of rg_train_batch when the caller passes no out_rgb
+    float* train_rgb = nullptr;
+    int64_t train_rgb_cap = 0;
 };
 
 namespace {
@@ -78,7 +81,6 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->hit_depth != 8 && c->hit_depth != 16 && c->hit_depth != 32)
         return fail(h, RG_ERR_ARG, "hit_depth must be 8, 16 or 32");
     if (c->seg_slabs < 1 || c->seg_slabs > 4096) return fail(h, RG_ERR_ARG, "seg_slabs in 1..4096");
-    if (c->chunk != 64 && c->chunk != 128 && c->chunk != 256) return fail(h, RG_ERR_ARG, "chunk must be 64, 128 or 256");
     return RG_OK;
 }
 
@@ -232,7 +234,8 @@ int rg_destroy(rg_handle* h) {
     void* ptrs[] = {h->geo, h->brec, h->leafbox, h->acc, h->box, h->counters, h->bvh.keys[0], h->bvh.keys[1],
                     h->bvh.vals[0], h->bvh.vals[1], h->bvh.hist, h->bvh.perm, h->bvh.iperm, h->bvh.nodes,
                     h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
-                    h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb};
+                    h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb,
+                    h->train_rgb};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete h;
@@ -330,8 +333,8 @@ int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pix
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
     const rg::DevGrad gr = dev_grad(h);
-    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, ray_colors, dL_dC, gr,
-                                  dev_stats(h), s));
+    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1,
+                                  gr, dev_stats(h), s));
     return RG_OK;
 }
 
@@ -372,8 +375,8 @@ int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colo
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::DevGrad gr = dev_grad(h);
-    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, ray_colors, dL_dC, gr,
-                                  dev_stats(h), s));
+    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1,
+                                  gr, dev_stats(h), s));
     return RG_OK;
 }
 
@@ -387,8 +390,19 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::DevGrad gr = dev_grad(h);
-    HIPCHK(h, rg::launch_train(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, h->cfg.chunk, target, loss_kind,
-                               out_rgb, out_loss, gr, dev_stats(h), s));
+    float* rgb = out_rgb;
+    if (!rgb) {
+        if (h->train_rgb_cap < b->num_rays) {
+            if ((rc = dalloc(h, &h->train_rgb, 3 * b->num_rays))) return rc;
+            h->train_rgb_cap = b->num_rays;
+        }
+        rgb = h->train_rgb;
+    }
+    const rg::DevScene sc = dev_scene(h);
+    const rg::DevParams p = dev_params(h);
+    HIPCHK(h, rg::launch_forward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, nullptr, dev_stats(h), s));
+    HIPCHK(h, rg::launch_backward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, target, loss_kind, 0, dev_grad(h),
+                                  dev_stats(h), s));
     if (out_loss)
         HIPCHK(h, hipMemcpyAsync(out_loss, h->counters + 6, sizeof(float), hipMemcpyDeviceToDevice, s));
     return RG_OK;

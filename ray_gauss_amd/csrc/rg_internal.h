@@ -62,7 +62,7 @@ struct DevGrad {
 };
 
 // Counters (device, 16): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 6 loss (float),
-// 7 debug scratch, 8..15 per-XCD work queues
+// 7 debug scratch, 8..15 unused
 struct DevStats {
     unsigned long long* c;
 };
@@ -115,10 +115,8 @@ struct RayJob {
 hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth, float* out_rgb,
                           float* out_depth, float* out_opacity, DevStats st, hipStream_t s);
 hipError_t launch_backward(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth,
-                           const float* ray_colors, const float* dLdC, DevGrad gr, DevStats st, hipStream_t s);
-hipError_t launch_train(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth, int chunk,
-                        const float* target, int loss_kind, float* out_rgb, float* out_loss, DevGrad gr, DevStats st,
-                        hipStream_t s);
+                           const float* ray_colors, const float* dLdC, const float* target, int loss_kind,
+                           int count_stats, DevGrad gr, DevStats st, hipStream_t s);
 hipError_t launch_reset_counters(DevStats st, hipStream_t s);
 hipError_t launch_debug_hits(const DevScene& sc, const DevParams& p, const RayJob& job, int max_slabs, int max_pairs,
                              int32_t* pairs, int64_t* counts, float* ro, float* rd, float* te, float* tx,

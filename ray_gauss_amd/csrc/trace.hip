@@ -1,5 +1,5 @@
-// Ray kernels of the gfx950 ray caster: forward render, recompute-based backward, fused
-// train step, and the hit-set export used for verification.
+// Ray kernels of the gfx950 ray caster: forward render, recompute-based backward (optionally
+// forming a per-pixel loss gradient itself), and the hit-set export used for verification.
 //
 // Per ray (one lane), the reference's per-slab loop (cuda_train/forward/gaussians_aabb.cu:521-612,
 // backward/gaussians_aabb.cu:505-775) is kept exactly -- slab grid tb accumulated in FP32 from
@@ -23,7 +23,8 @@
 // Gradient scatter is wave-cooperative: every lane's list is sorted by Morton slot, so after each
 // lane has prepared one slab, the 64 lanes (an 8x8 pixel tile) walk the union of their hit sets in
 // increasing slot order (a wave-wide min per step); the lanes holding the current Gaussian compute
-// their contributions together, the wave sums them, and one lane issues the 14 atomics.
+// their contributions together, and the wave sums them and issues the 14 atomic adds as one
+// 14-lane instruction.
 
 #include <hip/hip_runtime.h>
 
@@ -39,7 +40,7 @@ constexpr int kWave = 64;
 constexpr int kMaxSeg = 4096;
 constexpr int kGradVals = 14;
 
-enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6, C_QUEUE0 = 8 };
+enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6 };
 
 struct RayOut {
     float C[3];
@@ -283,6 +284,49 @@ __device__ __forceinline__ void grad_scatter(const float* v, int slot, int view,
     gadd(dr + 2, v[13], local);
 }
 
+// Butterfly step of the reduce-scatter below: the lanes with `upper` keep the upper half of
+// their H values and receive the partner's (lane ^ O) upper half; the others keep the lower.
+template <int H, int O>
+__device__ __forceinline__ void rs_step(const float* in, float* out, bool upper) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        const float keep = upper ? in[j + H] : in[j];
+        const float send = upper ? in[j] : in[j + H];
+        out[j] = keep + __shfl_xor(send, O, 64);
+    }
+}
+
+// Wave-wide sum of the 14 gradient values of one Gaussian (all lanes hold a contribution, zero
+// for lanes without one) followed by ONE 14-lane atomic instruction.  The sums are formed by a
+// reduce-scatter: after steps over lane bits 5..2 lane l holds value bitrev4(l >> 2) summed over
+// 16 lanes, and two more steps over bits 1..0 complete it (17 exchanges instead of 14 x 6).
+__device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int view, const DevScene& sc,
+                                                  const DevGrad& gr, int lane) {
+    float a[16], b[8], c[4], d[2];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = k < kGradVals ? v[k] : 0.0f;
+    rs_step<8, 32>(a, b, (lane & 32) != 0);
+    rs_step<4, 16>(b, c, (lane & 16) != 0);
+    rs_step<2, 8>(c, d, (lane & 8) != 0);
+    float x = ((lane & 4) != 0) ? d[1] : d[0];
+    x += __shfl_xor(((lane & 4) != 0) ? d[0] : d[1], 4, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    const int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+    if ((lane & 3) == 0 && idx < kGradVals) {
+        const bool local = gr.copies > 1;
+        const int copy = local ? xcc_id() : 0;
+        float* dst;
+        if (idx < 11) {
+            dst = gr.acc + (size_t)copy * gr.acc_stride + (size_t)kAccStride * slot + (idx < 7 ? idx : idx + 1);
+        } else {
+            dst = reinterpret_cast<float*>(gr.drgb + (size_t)copy * gr.drgb_stride + (size_t)view * sc.n + slot) +
+                  (idx - 11);
+        }
+        gadd(dst, x, local);
+    }
+}
+
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }
 
 // LBVH traversal of the lanes executing the call: wave-uniform packet walk or per-lane walk.
@@ -392,6 +436,20 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
     }
 }
 
+// Whether the lane's next slab needs a new gather because its list is spent (a lane crossing an
+// empty segment does not count: it only steps to the segment end).
+__device__ __forceinline__ bool needs_gather(const SegState& s, const Ray& ray, const DevParams& p) {
+    const float slab = p.dt * (float)p.spb;
+    return !s.valid || (!s.direct && s.cnt > 0 && fminf(ray.texit, add_exact(s.tb, slab)) > s.seg_hi);
+}
+
+// Wave-synchronous gathers: when one lane executing the call must gather, all of them gather
+// anew from their current slab, so that one packet traversal serves every lane.  The gathered
+// lists hold the same hit sets, so the result does not change.
+__device__ __forceinline__ void sync_gathers(SegState& s, const Ray& ray, const DevParams& p) {
+    if (__ballot(needs_gather(s, ray, p)) != 0ULL) s.valid = false;
+}
+
 __device__ __forceinline__ bool advance_slab(SegState& s, const DevParams& p) {
     const float nt = add_exact(s.tb, p.dt * (float)p.spb);
     if (s.direct) {
@@ -476,6 +534,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
     seg_init(s, ray, p);
     const float first_tb = ray.tenter;
     while (o.T > p.t_eps) {
+        sync_gathers(s, ray, p);
         float tmin, tmax;
         if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) break;
         const float tb = s.tb;
@@ -539,9 +598,13 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
         int cur_cnt = 0;
         if (alive && !pend) {
-            if (!(T > p.t_eps)) alive = false;
-            else if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) alive = false;
-            else pend = true;
+            if (!(T > p.t_eps)) {
+                alive = false;
+            } else {
+                sync_gathers(s, ray, p);
+                if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) alive = false;
+                else pend = true;
+            }
         }
         // depth synchronisation: the wave's merged scatter only pays off when the lanes' current
         // slabs are at similar depth (similar Gaussians), so lanes ahead of the laggard wait
@@ -637,15 +700,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 grad_vals<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, p, Aq, Bq, Eq, dL, sc, v);
                 ++e;
             }
-            const unsigned long long bal = __ballot(act);
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
-            if (__popcll(bal) == 1) {
-                if (act) grad_scatter(v, mslot, mview, sc, gr);
-            } else {
-#pragma unroll
-                for (int k = 0; k < kGradVals; ++k) v[k] = wave_sum64(v[k]);
-                if (lane == __ffsll((long long)bal) - 1) grad_scatter(v, mslot, mview, sc, gr);
-            }
+            grad_scatter_wave(v, mslot, mview, sc, gr, lane);
         }
     }
 }
@@ -700,9 +756,14 @@ __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, Ray
     flush_counters(ctr, st);
 }
 
+// Backward over a batch.  dL/dC comes from dLdC, or -- with target != nullptr -- from the
+// per-pixel loss of the rendered colours against target (loss_kind 0: L2 sum (C-t)^2, 1: L1
+// sum |C-t|), whose value is summed into the loss counter.  count_stats = 0 leaves the
+// traversal counters to the forward of the same step.
 template <int SPB, int CAP>
-__global__ void __launch_bounds__(kWave) k_backward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) k_backward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
                                                     const float* __restrict__ ray_colors, const float* __restrict__ dLdC,
+                                                    const float* __restrict__ target, int loss_kind, int count_stats,
                                                     DevGrad gr, DevStats st) {
     __shared__ IList<CAP> L;
     const int lane = threadIdx.x;
@@ -713,91 +774,42 @@ __global__ void __launch_bounds__(kWave) k_backward(DevScene sc, DevParams p, Ra
     bool active = false;
     Ray ray = {};
     float dL[3] = {0.0f, 0.0f, 0.0f}, Cf[3] = {0.0f, 0.0f, 0.0f};
+    float loss = 0.0f;
     int view = 0;
     if (item < num_items && resolve(job, item, q)) {
-        dL[0] = dLdC[3 * q.out];
-        dL[1] = dLdC[3 * q.out + 1];
-        dL[2] = dLdC[3 * q.out + 2];
+        Cf[0] = ray_colors[3 * q.out];
+        Cf[1] = ray_colors[3 * q.out + 1];
+        Cf[2] = ray_colors[3 * q.out + 2];
+        if (target) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float e = Cf[c] - target[3 * q.out + c];
+                if (loss_kind == 1) {
+                    loss += fabsf(e);
+                    dL[c] = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
+                } else {
+                    loss += e * e;
+                    dL[c] = 2.0f * e;
+                }
+            }
+        } else {
+            dL[0] = dLdC[3 * q.out];
+            dL[1] = dLdC[3 * q.out + 1];
+            dL[2] = dLdC[3 * q.out + 2];
+        }
         if (dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f) {
             active = true;
             view = q.view;
-            Cf[0] = ray_colors[3 * q.out];
-            Cf[1] = ray_colors[3 * q.out + 1];
-            Cf[2] = ray_colors[3 * q.out + 2];
             const rg_camera cam = job.cams[q.view];
             ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
         }
     }
     backward_wave<SPB, CAP>(sc, p, active, ray, sc.rgb + (size_t)view * sc.n, Cf, dL, L, lane, gr, view, ctr);
-    flush_counters(ctr, st);
-}
-
-// Fused train step, persistent: waves pull chunks of `chunk` rays from device queues; each ray
-// is rendered, its per-pixel loss gradient formed, and back-propagated by the same wave.
-template <int SPB, int CAP>
-__global__ void __launch_bounds__(kWave) k_train(DevScene sc, DevParams p, RayJob job, int64_t num_items, int chunk,
-                                                 const float* __restrict__ target, int loss_kind,
-                                                 float* __restrict__ out_rgb, DevGrad gr, DevStats st) {
-    __shared__ IList<CAP> L;
-    const int lane = threadIdx.x;
-    Counters ctr, dummy;
-    float loss = 0.0f;
-    // The work items are split into 8 contiguous ranges with one queue each; a wave drains the
-    // queue of the XCD it runs on first (that XCD's L2 then serves a compact part of the scene)
-    // and then steals from the others.  Every queue is drained, so the XCD id only affects speed.
-    const int xcc = xcc_id();
-    const int64_t per = (num_items + 7) / 8;
-    for (int k = 0; k < 8; ++k) {
-        const int qid = (xcc + k) & 7;
-        const int64_t q0 = (int64_t)qid * per;
-        const int64_t q1 = min(num_items, q0 + per);
-        while (q0 < q1) {
-            int64_t base = 0;
-            if (lane == 0) base = (int64_t)atomicAdd(&st.c[C_QUEUE0 + qid], (unsigned long long)chunk);
-            base = __shfl(base, 0, 64);
-            if (q0 + base >= q1) break;
-            for (int j = 0; j < chunk; j += kWave) {
-                const int64_t item = q0 + base + j + lane;
-                RayRef q;
-                bool act = false;
-                Ray ray = {};
-                float C[3] = {0.0f, 0.0f, 0.0f}, dL[3] = {0.0f, 0.0f, 0.0f};
-                int view = 0;
-                if (item < q1 && resolve(job, item, q)) {
-                    const rg_camera cam = job.cams[q.view];
-                    ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
-                    view = q.view;
-                    RayOut o;
-                    forward_ray<SPB, CAP>(sc, p, ray, sc.rgb + (size_t)view * sc.n, L, lane, ctr, o);
-                    C[0] = o.C[0] + o.T * p.bg;
-                    C[1] = o.C[1] + o.T * p.bg;
-                    C[2] = o.C[2] + o.T * p.bg;
-                    if (out_rgb) {
-                        out_rgb[3 * q.out + 0] = C[0];
-                        out_rgb[3 * q.out + 1] = C[1];
-                        out_rgb[3 * q.out + 2] = C[2];
-                    }
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const float e = C[c] - target[3 * q.out + c];
-                        if (loss_kind == 1) {
-                            loss += fabsf(e);
-                            dL[c] = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
-                        } else {
-                            loss += e * e;
-                            dL[c] = 2.0f * e;
-                        }
-                    }
-                    act = dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f;
-                }
-                backward_wave<SPB, CAP>(sc, p, act, ray, sc.rgb + (size_t)view * sc.n, C, dL, L, lane, gr, view,
-                                        dummy);
-            }
-        }
+    if (count_stats) flush_counters(ctr, st);
+    if (target) {
+        const float ls = wave_sum64(loss);
+        if (lane == 0 && ls != 0.0f) atomicAdd(reinterpret_cast<float*>(&st.c[C_LOSS]), ls);
     }
-    flush_counters(ctr, st);
-    const float ls = wave_sum64(loss);
-    if (lane == 0 && ls != 0.0f) atomicAdd(reinterpret_cast<float*>(&st.c[C_LOSS]), ls);
 }
 
 // Verification: per-slab hit sets from the same traversal + predicate (every slab to texit).
@@ -884,32 +896,14 @@ hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& 
 }
 
 hipError_t launch_backward(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth,
-                           const float* ray_colors, const float* dLdC, DevGrad gr, DevStats st, hipStream_t s) {
+                           const float* ray_colors, const float* dLdC, const float* target, int loss_kind,
+                           int count_stats, DevGrad gr, DevStats st, hipStream_t s) {
     const int64_t items = job.num_rays;
     if (items <= 0) return hipSuccess;
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
     RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, k_backward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
-                items, ray_colors, dLdC, gr, st);
-    return hipGetLastError();
-}
-
-hipError_t launch_train(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth, int chunk,
-                        const float* target, int loss_kind, float* out_rgb, float* out_loss, DevGrad gr, DevStats st,
-                        hipStream_t s) {
-    (void)out_loss;
-    const int64_t items = job.num_rays;
-    if (items <= 0) return hipSuccess;
-    int dev = 0;
-    hipGetDevice(&dev);
-    int cus = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) cus = 256;
-    const int64_t want = (items + chunk - 1) / chunk;
-    int64_t nb = (int64_t)cus * 16;
-    if (nb > want) nb = want;
-    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, k_train, dim3((unsigned)nb), dim3(kWave), 0, s, sc, p, job, items,
-                chunk, target, loss_kind, out_rgb, gr, st);
+                items, ray_colors, dLdC, target, loss_kind, count_stats, gr, st);
     return hipGetLastError();
 }
 

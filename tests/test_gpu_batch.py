@@ -1,4 +1,4 @@
-"""Batched multi-view path, fused train kernel, hipGraph replay and the drop-in autograd boundary
+"""Batched multi-view path, train step (forward + loss/backward), hipGraph replay and the drop-in autograd boundary
 on the GPU, each checked against the per-view path or the CPU oracle."""
 
 import math
@@ -128,7 +128,7 @@ def test_xcd_local_gradients_equal_device_scope(blobs, gpu_device):
     target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
     out = {}
     for xcd in (False, True):
-        rc, t = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(16, 8, 128, xcd))
+        rc, t = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(16, 8, xcd_grads=xcd))
         rc.train_batch(rv, rp, target, 0)
         out[xcd] = rc.finish_grads()
     for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis"):
@@ -138,7 +138,7 @@ def test_xcd_local_gradients_equal_device_scope(blobs, gpu_device):
 def test_autotune_picks_a_valid_config(blobs):
     sc, prm, cams, pixels, rc, t, rv, rp, offs = blobs
     target = torch.zeros((rv.shape[0], 3), device=rv.device)
-    cfg, table = autotune(rc, rv, rp, target, depths=(8, 16), segs=(8,), chunks=(64, 128), reps=1, save=False)
+    cfg, table = autotune(rc, rv, rp, target, depths=(8, 16), segs=(8,), packets=(True, False), reps=1, save=False)
     assert len(table) == 4 and cfg.hit_depth in (8, 16)
     rc.set_tune(TuneConfig())
 
