@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-view", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="run the batched step eagerly instead of as a HIP graph")
-    ap.add_argument("--hit-depth", type=int, default=16)
+    ap.add_argument("--hit-depth", type=int, default=8)
     ap.add_argument("--seg-slabs", type=int, default=8)
     ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -47,8 +47,6 @@ typedef struct rg_config {
     int32_t hit_depth;         /* per-ray interval list capacity: 8, 16 or 32                    */
     int32_t seg_slabs;         /* initial slabs per interval gather: 1..4096                     */
     int32_t max_hits_per_slab; /* >0: count slabs whose hit set exceeds it (max_prim_slice)      */
-    int32_t xcd_local_grads;   /* 1: one gradient copy per XCD (L2-local atomics), summed at the
-                                  end; 0: one copy, device-scope atomics                          */
     int32_t packet_traversal;  /* 1: the lanes of a wavefront walk the LBVH together (one node
                                   sequence, per-lane box tests); 0: per-lane walks                */
 } rg_config;

@@ -19,8 +19,7 @@ class Config(C.Structure):
     _fields_ = [("dt", C.c_float), ("samples_per_slab", C.c_int32), ("t_eps", C.c_float), ("sigma_thr", C.c_float),
                 ("t_near", C.c_float), ("t_far", C.c_float), ("white_background", C.c_int32),
                 ("exact_sh_grad", C.c_int32), ("hit_depth", C.c_int32), ("seg_slabs", C.c_int32),
-                ("max_hits_per_slab", C.c_int32), ("xcd_local_grads", C.c_int32),
-                ("packet_traversal", C.c_int32)]
+                ("max_hits_per_slab", C.c_int32), ("packet_traversal", C.c_int32)]
 
 
 class CameraS(C.Structure):

@@ -20,13 +20,11 @@ from .cameras import launch_record_from
 
 @dataclass
 class TuneConfig:
-    """Launch tuning: interval-list depth per ray, initial slabs per gather, whether gradients
-    accumulate in one copy per XCD (L2-local atomics), and whether a wavefront's lanes walk the
-    LBVH as one packet."""
+    """Launch tuning: interval-list depth per ray, initial slabs per gather, and whether a
+    wavefront's lanes walk the LBVH as one packet."""
 
     hit_depth: int = 8
     seg_slabs: int = 8
-    xcd_grads: bool = True
     packet: bool = True
 
 
@@ -104,7 +102,6 @@ class RayCaster:
         c.hit_depth = int(self.tune.hit_depth)
         c.seg_slabs = int(self.tune.seg_slabs)
         c.max_hits_per_slab = int(self.max_hits_per_slab)
-        c.xcd_local_grads = int(bool(self.tune.xcd_grads))
         c.packet_traversal = int(bool(self.tune.packet))
         return c
 

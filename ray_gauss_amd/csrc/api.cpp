@@ -43,7 +43,6 @@ struct rg_handle {
 
 namespace {
 
-constexpr int kGradCopies = 8;  // one gradient copy per XCD
 
 int fail(rg_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -92,7 +91,7 @@ int ensure_gaussian_capacity(rg_handle* h, int64_t n) {
     if ((rc = dalloc(h, &h->geo, 4 * cap))) return rc;
     if ((rc = dalloc(h, &h->brec, 2 * cap))) return rc;
     if ((rc = dalloc(h, &h->leafbox, 2 * cap))) return rc;
-    if ((rc = dalloc(h, &h->acc, (int64_t)kGradCopies * rg::kAccStride * cap))) return rc;
+    if ((rc = dalloc(h, &h->acc, (int64_t)rg::kAccStride * cap))) return rc;
     rg::BvhBuffers& b = h->bvh;
     if ((rc = dalloc(h, &b.keys[0], cap))) return rc;
     if ((rc = dalloc(h, &b.keys[1], cap))) return rc;
@@ -126,9 +125,9 @@ int ensure_view_capacity(rg_handle* h, int views) {
     }
     if (need > h->rgb_cap || !h->rgb) {
         if ((rc = dalloc(h, &h->rgb, need))) return rc;
-        if ((rc = dalloc(h, &h->drgb, kGradCopies * need))) return rc;
+        if ((rc = dalloc(h, &h->drgb, need))) return rc;
         h->rgb_cap = need;
-        HIPCHK(h, hipMemset(h->drgb, 0, sizeof(float4) * kGradCopies * need));
+        HIPCHK(h, hipMemset(h->drgb, 0, sizeof(float4) * need));
     }
     return RG_OK;
 }
@@ -190,9 +189,7 @@ rg::DevGrad dev_grad(rg_handle* h) {
     rg::DevGrad g;
     g.acc = h->acc;
     g.drgb = h->drgb;
-    g.copies = h->cfg.xcd_local_grads ? kGradCopies : 1;
-    g.acc_stride = (int64_t)rg::kAccStride * h->cap;
-    g.drgb_stride = h->rgb_cap;
+    g.num_views = h->num_views;
     return g;
 }
 
@@ -276,7 +273,7 @@ int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* strea
         HIPCHK(h, rg::launch_iota(n, h->bvh.perm, h->bvh.iperm, s));
         HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
         HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
-        HIPCHK(h, rg::launch_zero(h->acc, (int64_t)kGradCopies * rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
+        HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
     }
     HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
     HIPCHK(h, rg::launch_lbvh_refit(n, h->leafbox, h->bvh, s));
@@ -411,8 +408,8 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
 int rg_zero_grads(rg_handle* h, void* stream) {
     if (!h) return RG_ERR_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->acc) HIPCHK(h, rg::launch_zero(h->acc, (int64_t)kGradCopies * rg::kAccStride * h->cap, s));
-    if (h->drgb) HIPCHK(h, rg::launch_zero(reinterpret_cast<float*>(h->drgb), 4 * kGradCopies * h->rgb_cap, s));
+    if (h->acc) HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * h->cap, s));
+    if (h->drgb) HIPCHK(h, rg::launch_zero(reinterpret_cast<float*>(h->drgb), 4 * h->rgb_cap, s));
     return RG_OK;
 }
 

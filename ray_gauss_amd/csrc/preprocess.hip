@@ -221,15 +221,10 @@ __global__ void __launch_bounds__(kBlock) k_finish(int n, int deg, int L, const 
     const float mx = pos[3 * g], my = pos[3 * g + 1], mz = pos[3 * g + 2];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
     for (int v = 0; v < num_views; ++v) {
-        float dL[3] = {0.0f, 0.0f, 0.0f};
-        for (int c = 0; c < gr.copies; ++c) {
-            float4* p = gr.drgb + (size_t)c * gr.drgb_stride + (size_t)v * n + slot;
-            const float4 d4 = *p;
-            *p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            dL[0] += d4.x;
-            dL[1] += d4.y;
-            dL[2] += d4.z;
-        }
+        float4* p = gr.drgb + (size_t)v * n + slot;
+        const float4 d4 = *p;
+        *p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float dL[3] = {d4.x, d4.y, d4.z};
         if (dL[0] == 0.0f && dL[1] == 0.0f && dL[2] == 0.0f) continue;  // sh_utils.cu:129-131
         const rg_camera& cam = cams[v];
         const float d0x = mx - cam.eye[0], d0y = my - cam.eye[1], d0z = mz - cam.eye[2];
@@ -306,17 +301,11 @@ __global__ void __launch_bounds__(kBlock) k_finish(int n, int deg, int L, const 
         }
     }
     const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 a0 = z4, a1 = z4, a2 = z4;
-    for (int c = 0; c < gr.copies; ++c) {
-        float4* a = reinterpret_cast<float4*>(gr.acc + (size_t)c * gr.acc_stride + (size_t)kAccStride * slot);
-        const float4 b0 = a[0], b1 = a[1], b2 = a[2];
-        a[0] = z4;
-        a[1] = z4;
-        a[2] = z4;
-        a0 = make_float4(a0.x + b0.x, a0.y + b0.y, a0.z + b0.z, a0.w + b0.w);
-        a1 = make_float4(a1.x + b1.x, a1.y + b1.y, a1.z + b1.z, a1.w + b1.w);
-        a2 = make_float4(a2.x + b2.x, a2.y + b2.y, a2.z + b2.z, a2.w + b2.w);
-    }
+    float4* a = reinterpret_cast<float4*>(gr.acc + (size_t)kAccStride * slot);
+    const float4 a0 = a[0], a1 = a[1], a2 = a[2];
+    a[0] = z4;
+    a[1] = z4;
+    a[2] = z4;
     out.positions[3 * g + 0] = a0.x + dpx;
     out.positions[3 * g + 1] = a0.y + dpy;
     out.positions[3 * g + 2] = a0.z + dpz;

@@ -151,9 +151,12 @@ __device__ __forceinline__ bool overlaps(float tmin, float tmax, float r1, float
 // Conservative ray-segment / box test (boxes are padded at build time).
 __device__ __forceinline__ bool box_hit(const float4 lo, const float4 hi, const Ray& r, float ix, float iy, float iz,
                                         float ta, float tb) {
-    const float ax = (lo.x - r.ox) * ix, bx = (hi.x - r.ox) * ix;
-    const float ay = (lo.y - r.oy) * iy, by = (hi.y - r.oy) * iy;
-    const float az = (lo.z - r.oz) * iz, bz = (hi.z - r.oz) * iz;
+    // one FMA per plane (-o*inv is loop-invariant); the rounding differs from (lo - o) * inv by a
+    // few ulp of t, far inside the box padding.  A zero direction component gives NaN planes,
+    // which fminf/fmaxf ignore: the test then only widens.
+    const float ax = fmaf(lo.x, ix, -r.ox * ix), bx = fmaf(hi.x, ix, -r.ox * ix);
+    const float ay = fmaf(lo.y, iy, -r.oy * iy), by = fmaf(hi.y, iy, -r.oy * iy);
+    const float az = fmaf(lo.z, iz, -r.oz * iz), bz = fmaf(hi.z, iz, -r.oz * iz);
     const float tn = fmaxf(fmaxf(ta, fminf(ax, bx)), fmaxf(fminf(ay, by), fminf(az, bz)));
     const float tf = fminf(fminf(tb, fmaxf(ax, bx)), fminf(fmaxf(ay, by), fmaxf(az, bz)));
     return tn <= tf;

@@ -50,15 +50,11 @@ struct DevParams {
     int packet;      // 1: wave-uniform packet traversal, 0: per-lane traversal
 };
 
-// Gradient accumulators.  With copies == 8 each XCD adds into its own copy with workgroup-scope
-// atomics, which complete in that XCD's L2; the finish kernel sums the copies.  With copies == 1
-// all waves add into one buffer with device-scope atomics.
+// Gradient accumulators (device-scope atomic adds; the finish kernel reads and clears them).
 struct DevGrad {
-    float* acc;            // copies x (16 floats per slot)
-    float4* drgb;          // copies x (views x slots)
-    int copies;            // 1 or 8
-    int64_t acc_stride;    // floats between copies
-    int64_t drgb_stride;   // float4 between copies
+    float* acc;     // 16 floats per slot
+    float4* drgb;   // views x slots
+    int num_views;  // rows of drgb
 };
 
 // Counters (device, 16): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 6 loss (float),

@@ -76,14 +76,16 @@ struct Counters {
     unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0;
 };
 
-// XCD (accelerator complex die) the wave runs on: hardware register XCC_ID, low 3 bits.
-__device__ __forceinline__ int xcc_id() { return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 7); }
+// FP32 gradient add (device scope, relaxed: the result is read by a later kernel).
+__device__ __forceinline__ void gadd(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-// FP32 gradient add: workgroup scope keeps the atomic in the XCD's L2 (per-XCD copies);
-// device scope otherwise.
-__device__ __forceinline__ void gadd(float* p, float v, bool local) {
-    if (local) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else atomicAdd(p, v);
+// Address of gradient value k (0..13) of a slot: 0-2 d mu, 3 d sigma, 4-6 d s, 7-10 d q in the
+// slot's accumulator record (offsets 0-6, 8-11), 11-13 the view's colour gradient.
+__device__ __forceinline__ float* grad_addr(const DevGrad& gr, int n, int slot, int view, int k) {
+    if (k < 11) return gr.acc + (size_t)kAccStride * slot + (k < 7 ? k : k + 1);
+    return reinterpret_cast<float*>(gr.drgb + (size_t)view * n + slot) + (k - 11);
 }
 
 __device__ __forceinline__ float wave_sum64(float v) {
@@ -261,27 +263,33 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
     v[13] = dL[2] * sc_;
 }
 
-// 14 FP32 atomic adds of one Gaussian's (summed) gradient contribution.
-__device__ __forceinline__ void grad_scatter(const float* v, int slot, int view, const DevScene& sc,
-                                             const DevGrad& gr) {
-    const bool local = gr.copies > 1;
-    const int copy = local ? xcc_id() : 0;
-    float* acc = gr.acc + (size_t)copy * gr.acc_stride + (size_t)kAccStride * slot;
-    gadd(acc + 0, v[0], local);
-    gadd(acc + 1, v[1], local);
-    gadd(acc + 2, v[2], local);
-    gadd(acc + 3, v[3], local);
-    gadd(acc + 4, v[4], local);
-    gadd(acc + 5, v[5], local);
-    gadd(acc + 6, v[6], local);
-    gadd(acc + 8, v[7], local);
-    gadd(acc + 9, v[8], local);
-    gadd(acc + 10, v[9], local);
-    gadd(acc + 11, v[10], local);
-    float* dr = reinterpret_cast<float*>(gr.drgb + (size_t)copy * gr.drgb_stride + (size_t)view * sc.n + slot);
-    gadd(dr + 0, v[11], local);
-    gadd(dr + 1, v[12], local);
-    gadd(dr + 2, v[13], local);
+// Staging area of the wave-merged gradient sums in LDS.  Global atomics are issued in bursts of
+// kStage records: every vector memory operation of a wave completes in order, so an atomic
+// issued between two loads makes the wave wait for its round trip at the next load; a burst
+// pays that wait once per kStage records instead of once per record.
+constexpr int kStage = 16;
+struct GradStage {
+    float val[kStage][16];
+    int slot[kStage];
+    int view[kStage];
+};
+
+// Issues the first cnt staged records: four records per instruction (lanes 16e..16e+13).  The
+// records were written by other lanes: the workgroup is one wavefront, and the barriers order
+// the LDS writes before these reads and these reads before the next records' writes.
+__device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScene& sc, const DevGrad& gr,
+                                            int lane) {
+    __syncthreads();
+    const int j = lane & 15;
+    for (int e0 = 0; e0 < cnt; e0 += 4) {
+        const int e = e0 + (lane >> 4);
+        if (e < cnt && j < kGradVals) {
+            const int slot = G.slot[e], view = G.view[e];
+            if ((unsigned)slot < (unsigned)sc.n && (unsigned)view < (unsigned)gr.num_views)
+                gadd(grad_addr(gr, sc.n, slot, view, j), G.val[e][j]);
+        }
+    }
+    __syncthreads();
 }
 
 // Butterfly step of the reduce-scatter below: the lanes with `upper` keep the upper half of
@@ -297,11 +305,12 @@ __device__ __forceinline__ void rs_step(const float* in, float* out, bool upper)
 }
 
 // Wave-wide sum of the 14 gradient values of one Gaussian (all lanes hold a contribution, zero
-// for lanes without one) followed by ONE 14-lane atomic instruction.  The sums are formed by a
-// reduce-scatter: after steps over lane bits 5..2 lane l holds value bitrev4(l >> 2) summed over
-// 16 lanes, and two more steps over bits 1..0 complete it (17 exchanges instead of 14 x 6).
+// for lanes without one), appended to the staging area as record nst (flushed when full).  The
+// sums are formed by a reduce-scatter: after steps over lane bits 5..2 lane l holds value
+// bitrev4(l >> 2) summed over 16 lanes, and two more steps over bits 1..0 complete it (17
+// exchanges instead of 14 x 6).  Called by all 64 lanes together.
 __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int view, const DevScene& sc,
-                                                  const DevGrad& gr, int lane) {
+                                                  const DevGrad& gr, int lane, GradStage& G, int& nst) {
     float a[16], b[8], c[4], d[2];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = k < kGradVals ? v[k] : 0.0f;
@@ -313,17 +322,14 @@ __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int 
     x += __shfl_xor(x, 2, 64);
     x += __shfl_xor(x, 1, 64);
     const int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
-    if ((lane & 3) == 0 && idx < kGradVals) {
-        const bool local = gr.copies > 1;
-        const int copy = local ? xcc_id() : 0;
-        float* dst;
-        if (idx < 11) {
-            dst = gr.acc + (size_t)copy * gr.acc_stride + (size_t)kAccStride * slot + (idx < 7 ? idx : idx + 1);
-        } else {
-            dst = reinterpret_cast<float*>(gr.drgb + (size_t)copy * gr.drgb_stride + (size_t)view * sc.n + slot) +
-                  (idx - 11);
-        }
-        gadd(dst, x, local);
+    if ((lane & 3) == 0) G.val[nst][idx] = x;
+    if (lane == 0) {
+        G.slot[nst] = slot;
+        G.view[nst] = view;
+    }
+    if (++nst == kStage) {
+        stage_flush(G, kStage, sc, gr, lane);
+        nst = 0;
     }
 }
 
@@ -337,14 +343,15 @@ __device__ __forceinline__ void trav(const DevParams& p, const DevScene& sc, con
     else traverse(sc.nodes, sc.total_nodes, ray, ta, tb, on_leaf);
 }
 
-// Gathers the intervals overlapping [ta, tb] into the lane's list; returns the count, or
-// CAP + 1 on overflow.  Entries come out in DFS order of the tree = increasing Morton slot.
+// Gathers the intervals overlapping [ta, tb] of the Gaussians with slot > after into the lane's
+// list; returns the count, or CAP + 1 on overflow (the list then holds the first CAP).  Entries
+// come out in DFS order of the tree = increasing Morton slot.
 template <int CAP>
 __device__ __forceinline__ int gather(const DevScene& sc, const DevParams& p, const Ray& ray, float ta, float tb,
-                                      IList<CAP>& L,
-                                      int lane) {
+                                      IList<CAP>& L, int lane, int after = -1) {
     int cnt = 0;
     trav(p, sc, ray, ta, tb, [&](int slot) -> bool {
+        if (slot <= after) return true;
         float4 g0, g1, g2, g3;
         load_rec(sc.geo, slot, g0, g1, g2, g3);
         Isect I;
@@ -572,6 +579,64 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
     }
 }
 
+// Scatter of the slabs whose hit set overflowed the list (lanes with dsc), called by all 64
+// lanes together: a packet traversal over the lanes' slab intervals; at each leaf the lanes whose
+// slab holds the Gaussian compute its gradient contribution and the wave sums them into one
+// staged record per view (as in the list scatter), instead of 14 atomics per lane and hit.
+template <int SPB>
+__device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool dsc, const Ray& ray,
+                                    const float4* __restrict__ rgbv, float cur_tb, float cur_tmin, float cur_tmax,
+                                    const float* Aq, const float* Bq, const float* Eq, const float* dL, GradStage& G,
+                                    int& nst, int lane, const DevGrad& gr, int view) {
+    const float ix = 1.0f / ray.dx, iy = 1.0f / ray.dy, iz = 1.0f / ray.dz;
+    const int total = sc.total_nodes;
+    bool live = dsc;
+    int idx = 0;
+    while (idx < total && __ballot(live) != 0ULL) {
+        idx = __builtin_amdgcn_readfirstlane(idx);
+        const float4 lo = sc.nodes[2 * idx];
+        const float4 hi = sc.nodes[2 * idx + 1];
+        const int miss = __float_as_int(lo.w);
+        const int leaf = __float_as_int(hi.w);
+        if (miss <= idx) break;  // malformed link (uniform): never loop
+        const bool h = live && box_hit(lo, hi, ray, ix, iy, iz, cur_tmin, cur_tmax);
+        if (__ballot(h) == 0ULL) {
+            idx = miss;
+            continue;
+        }
+        if (leaf < 0) {
+            idx = idx + 1;
+            continue;
+        }
+        if (leaf >= sc.n) break;  // malformed leaf index (uniform)
+        float v[kGradVals];
+#pragma unroll
+        for (int k = 0; k < kGradVals; ++k) v[k] = 0.0f;
+        bool c = false;
+        if (h) {
+            float4 g0, g1, g2, g3;
+            load_rec(sc.geo, leaf, g0, g1, g2, g3);
+            Isect I;
+            if (intersect(g0, g1, g2, g3, ray, I) && overlaps(cur_tmin, cur_tmax, I.r1, I.r2)) {
+                refine_sampling(g0, g1, g2, g3, ray, I);
+                grad_vals<SPB>(I, leaf, g0, g1, g2, g3, rgbv[leaf], cur_tb, p, Aq, Bq, Eq, dL, sc, v);
+                c = true;
+            }
+        }
+        unsigned long long rem = __ballot(c);
+        while (rem != 0ULL) {  // one record per view among the contributing lanes (normally one)
+            const int vw = __builtin_amdgcn_readlane(view, __ffsll((long long)rem) - 1);
+            const bool mine = c && view == vw;
+            float w[kGradVals];
+#pragma unroll
+            for (int k = 0; k < kGradVals; ++k) w[k] = mine ? v[k] : 0.0f;
+            grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst);
+            rem &= ~__ballot(mine);
+        }
+        idx = miss;
+    }
+}
+
 // Backward of one ray per lane, called by all 64 lanes of the wave together (inactive lanes pass
 // active = false).  Each outer iteration every live lane prepares its next slab that has hits
 // (recomputed buffers, per-sample coefficients, T and Crem advanced); then the wave walks the
@@ -580,7 +645,8 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
 template <int SPB, int CAP>
 __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray,
                               const float4* __restrict__ rgbv, const float* Cfwd, const float* dL, IList<CAP>& L,
-                              int lane, const DevGrad& gr, int view, Counters& ctr) {
+                              GradStage& G, int lane, const DevGrad& gr, int view, Counters& ctr) {
+    int nst = 0;  // records in the staging area
     bool alive = active && (ray.tenter < ray.texit) && (add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter);
     float T = 1.0f;
     float Crem[3] = {Cfwd[0], Cfwd[1], Cfwd[2]};
@@ -594,7 +660,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
     bool pend = false;  // next_slab() has produced a slab not yet processed
     float tmin = 0.0f, tmax = 0.0f;
     while (__ballot(alive) != 0ULL) {
-        bool has = false;
+        // prepared slab: list mode (has: cur_cnt list entries) or per-slab traversal (dsc)
+        bool has = false, dsc = false;
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
         int cur_cnt = 0;
         if (alive && !pend) {
@@ -640,38 +707,30 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 }
                 if (nh > 0) {
                     slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
-                    if (s.direct) {
-                        // per-slab traversal: scatter per lane
-                        trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
-                            float4 g0, g1, g2, g3;
-                            load_rec(sc.geo, slot, g0, g1, g2, g3);
-                            Isect I;
-                            if (!intersect(g0, g1, g2, g3, ray, I)) return true;
-                            if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
-                            refine_sampling(g0, g1, g2, g3, ray, I);
-                            float v[kGradVals];
-                            grad_vals<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], tb, p, sig, cr, cg, dL, sc, v);
-                            grad_scatter(v, slot, view, sc, gr);
-                            return true;
-                        });
-                    } else {
 #pragma unroll
-                        for (int i = 0; i < SPB; ++i) {
-                            Aq[i] = sig[i];
-                            Bq[i] = cr[i];
-                            Eq[i] = cg[i];
-                        }
+                    for (int i = 0; i < SPB; ++i) {
+                        Aq[i] = sig[i];
+                        Bq[i] = cr[i];
+                        Eq[i] = cg[i];
+                    }
+                    cur_tb = tb;
+                    cur_tmin = tmin;
+                    cur_tmax = tmax;
+                    if (s.direct) {
+                        dsc = true;
+                    } else {
                         has = true;
-                        cur_tb = tb;
-                        cur_tmin = tmin;
-                        cur_tmax = tmax;
                         cur_cnt = s.cnt;
                     }
                 }
                 if (!advance_slab(s, p)) alive = false;
             }
         }
-        // converged: wave-merged scatter over the prepared slabs' hits
+        // converged: slabs whose hit set overflowed the list scatter from a packet traversal
+        if (__ballot(dsc) != 0ULL)
+            direct_scatter_wave<SPB>(sc, p, dsc, ray, rgbv, cur_tb, cur_tmin, cur_tmax, Aq, Bq, Eq, dL, G, nst, lane,
+                                     gr, view);
+        // wave-merged scatter over the prepared slabs' list hits
         int e = 0;
         while (true) {
             int my = INT_MAX;
@@ -701,9 +760,10 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 ++e;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
-            grad_scatter_wave(v, mslot, mview, sc, gr, lane);
+            grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst);
         }
     }
+    stage_flush(G, nst, sc, gr, lane);
 }
 
 // ------------------------------------------------------------------ kernels ------------------
@@ -766,6 +826,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
                                                     const float* __restrict__ target, int loss_kind, int count_stats,
                                                     DevGrad gr, DevStats st) {
     __shared__ IList<CAP> L;
+    __shared__ GradStage G;
     const int lane = threadIdx.x;
     const int64_t nb = gridDim.x;
     const int64_t item = block_item(blockIdx.x, nb) * kWave + lane;
@@ -804,7 +865,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
             ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
         }
     }
-    backward_wave<SPB, CAP>(sc, p, active, ray, sc.rgb + (size_t)view * sc.n, Cf, dL, L, lane, gr, view, ctr);
+    backward_wave<SPB, CAP>(sc, p, active, ray, sc.rgb + (size_t)view * sc.n, Cf, dL, L, G, lane, gr, view, ctr);
     if (count_stats) flush_counters(ctr, st);
     if (target) {
         const float ls = wave_sum64(loss);
