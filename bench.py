@@ -55,6 +55,21 @@ def backward_bytes(R, P):
     return 24.0 * R + 112.0 * P
 
 
+def measured_traffic(workload, kernel="k_backward"):
+    """HBM-side bytes per launch of `kernel` from the rocprofv3 summary committed for this workload
+    (FETCH_SIZE + WRITE_SIZE, each scaled by its reading for a device copy of known size); None
+    when no summary exists.  The summary names the commands it was made with."""
+    path = os.path.join(ROOT, "profiles", f"k_backward_{workload}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def forward_bytes(R, P):
     """The forward launch's share: colour write per ray (12 B), Gaussian read per pair (56 B)."""
     return 12.0 * R + 56.0 * P
@@ -223,6 +238,7 @@ def main():
     B_bwd = backward_bytes(R, P)
     achieved = B_bwd / (bk_ms * 1e-3) / 1e9
     B_step = step_bytes(R, P, sc.n, (sc.sh_degree + 1) ** 2, L, len(cams))
+    traffic, traffic_src = measured_traffic(args.workload)
     out = {
         "metric": "train rays/s (fwd+bwd) at 1M Gaussians, batched vs per-view; HBM GB/s",
         "value": R / dt_b,
@@ -246,8 +262,8 @@ def main():
         "counters": {"K_slab_hits": K, "P_pairs": P, "fallback_slabs": st["fallback_slabs"],
                      "gathers": st["gathers"]},
         "roofline": {"bound": "hbm", "kernel": "k_backward", "kernel_ms": bk_ms, "achieved": achieved,
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "algorithmic_bytes": B_bwd},
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_bytes": B_bwd},
         "forward_kernel": {"kernel": "k_forward", "kernel_ms": fk_ms,
                            "achieved_GBps": forward_bytes(R, P) / (fk_ms * 1e-3) / 1e9},
         "step_hbm_GBps": B_step / dt_b / 1e9,
