@@ -60,6 +60,18 @@ struct IList {
     float Cq[CAP][kWave];
 };
 
+// Bit e set for the list entries (e < cnt) whose interval meets the slab [tmin, tmax]: the
+// lanes then loop over their own hits only (in increasing entry = slot order), so a wave
+// iterates max-over-lanes(hits) times instead of max-over-lanes(cnt).
+template <int CAP>
+__device__ __forceinline__ unsigned int slab_mask(const IList<CAP>& L, int cnt, float tmin, float tmax, int lane) {
+    unsigned int m = 0u;
+#pragma unroll
+    for (int e = 0; e < CAP; ++e)
+        if (e < cnt && overlaps(tmin, tmax, L.r1[e][lane], L.r2[e][lane])) m |= 1u << e;
+    return m;
+}
+
 // Sampling quadratic of list entry e (as computed by intersect + refine_sampling at gather time).
 template <int CAP>
 __device__ __forceinline__ Isect entry_isect(const IList<CAP>& L, int e, int lane) {
@@ -566,8 +578,10 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
                 return true;
             });
         } else {
-            for (int e = 0; e < s.cnt; ++e) {
-                if (!overlaps(tmin, tmax, L.r1[e][lane], L.r2[e][lane])) continue;
+            unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
+            while (m != 0u) {
+                const int e = __builtin_ctz(m);
+                m &= m - 1u;
                 const int slot = L.g[e][lane];
                 eval_hit(slot, entry_isect(L, e, lane), sc.geo[4 * slot].w);
             }
@@ -697,8 +711,10 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         return true;
                     });
                 } else {
-                    for (int e = 0; e < s.cnt; ++e) {
-                        if (!overlaps(tmin, tmax, L.r1[e][lane], L.r2[e][lane])) continue;
+                    unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
+                    while (m != 0u) {
+                        const int e = __builtin_ctz(m);
+                        m &= m - 1u;
                         const int slot = L.g[e][lane];
                         accum_hit<SPB>(entry_isect(L, e, lane), sc.geo[4 * slot].w, rgbv[slot], tb, p, sig, cr, cg,
                                        cb);
