@@ -192,6 +192,46 @@ int rg_debug_hits(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* p
  * and every leaf appears exactly once; nonzero counts errors. */
 int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream);
 
+/* ---- optimiser step (handle-free; errors via rg_last_error(NULL)) ------------------------- */
+
+/* One parameter tensor of an Adam step with its torch.optim.Adam state (exp_avg, exp_avg_sq),
+ * all device pointers of n floats.  step = the state's step count AFTER this update. */
+typedef struct rg_adam_tensor {
+    float* param;
+    const float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    int64_t n;
+    double lr;
+    double beta1;
+    double beta2;
+    double eps;
+    int64_t step;
+} rg_adam_tensor;
+
+/* Adam update of up to 16 tensors in one launch.  Replaces the per-tensor torch.optim.Adam(eps=
+ * 1e-15) steps of the training loop (classes/optimizer.py:13-20 OptimizerManager.step,
+ * utils/optim_utils.py:4-83 define_optimizer_manager, scripts/train.py:121-124). */
+int rg_adam_step(const rg_adam_tensor* tensors, int32_t count, void* stream);
+
+/* Post-step parameter projections (scripts/train.py:168-183): densities clamped at 0,
+ * quaternions (n,4) normalised (a zero one becomes (1,0,0,0), classes/point_cloud.py:703-706),
+ * lobe axes (n,num_sg,3) normalised (a zero one becomes (0,0,1), :721-724), log-scales (n,3)
+ * clamped below by filter_3d (n, may be NULL) and above by max_scale.  NULL arrays are skipped. */
+typedef struct rg_projection {
+    int64_t n;
+    int32_t num_sg;
+    float* densities;
+    float* log_scales;
+    const float* filter_3d;
+    float max_scale;
+    float* quats;
+    int32_t normalize_quaternion;
+    float* lobe_axis;
+    int32_t normalize_lobe_axis;
+} rg_projection;
+int rg_project_params(const rg_projection* pr, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@ Public surface:
   RenderOptixSettings, RenderOptixFunction  -- drop-in autograd boundary (render_optix.py)
   optix_utils                               -- host-glue names used by the training loop
   RayCaster, TuneConfig                     -- direct handle on the HIP library
+  FusedAdam, step_all, project_params       -- optimiser step + post-step projections (optim.py)
   scenes, cameras                           -- synthetic workloads and pinhole cameras
 """
 
@@ -12,7 +13,8 @@ import importlib
 from .scenes import RenderParams  # noqa: F401
 
 _LAZY = {"RenderOptixSettings": "render", "RenderOptixFunction": "render", "render": "render",
-         "RayCaster": "caster", "TuneConfig": "caster"}
+         "RayCaster": "caster", "TuneConfig": "caster", "FusedAdam": "optim", "step_all": "optim",
+         "project_params": "optim"}
 
 
 def __getattr__(name):

@@ -46,6 +46,18 @@ class Stats(C.Structure):
                 ("fallback_slabs", C.c_int64), ("gathers", C.c_int64)]
 
 
+class AdamTensor(C.Structure):
+    _fields_ = [("param", C.c_void_p), ("grad", C.c_void_p), ("exp_avg", C.c_void_p), ("exp_avg_sq", C.c_void_p),
+                ("n", C.c_int64), ("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double), ("step", C.c_int64)]
+
+
+class Projection(C.Structure):
+    _fields_ = [("n", C.c_int64), ("num_sg", C.c_int32), ("densities", C.c_void_p), ("log_scales", C.c_void_p),
+                ("filter_3d", C.c_void_p), ("max_scale", C.c_float), ("quats", C.c_void_p),
+                ("normalize_quaternion", C.c_int32), ("lobe_axis", C.c_void_p), ("normalize_lobe_axis", C.c_int32)]
+
+
 EXPORTS = {
     "rg_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Config)]),
     "rg_destroy": (C.c_int, [C.c_void_p]),
@@ -68,6 +80,8 @@ EXPORTS = {
     "rg_debug_hits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rg_debug_check_bvh": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "rg_adam_step": (C.c_int, [C.POINTER(AdamTensor), C.c_int32, C.c_void_p]),
+    "rg_project_params": (C.c_int, [C.POINTER(Projection), C.c_void_p]),
 }
 
 

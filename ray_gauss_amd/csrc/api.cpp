@@ -42,6 +42,16 @@ struct rg_handle {
 };
 
 namespace {
+// Message of the last failed handle-free call (rg_adam_step, rg_project_params) on this thread.
+thread_local std::string g_free_error = "no error";
+
+int fail_free(int code, const std::string& msg) {
+    g_free_error = msg;
+    return code;
+}
+}  // namespace
+
+namespace {
 
 
 int fail(rg_handle* h, int code, const std::string& msg) {
@@ -249,7 +259,7 @@ int rg_set_config(rg_handle* h, const rg_config* cfg) {
     return RG_OK;
 }
 
-const char* rg_last_error(const rg_handle* h) { return h ? h->err.c_str() : "null handle"; }
+const char* rg_last_error(const rg_handle* h) { return h ? h->err.c_str() : g_free_error.c_str(); }
 
 int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* stream) {
     if (!h || !g) return fail(h, RG_ERR_ARG, "null argument");
@@ -488,6 +498,54 @@ int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream) {
     HIPCHK(h, hipMemcpyAsync(&e, d, sizeof(e), hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
     *errors = (int64_t)e;
+    return RG_OK;
+}
+
+int rg_adam_step(const rg_adam_tensor* tensors, int32_t count, void* stream) {
+    if (count < 0 || count > rg::kMaxAdamTensors) return fail_free(RG_ERR_ARG, "rg_adam_step: 0..16 tensors");
+    if (count > 0 && !tensors) return fail_free(RG_ERR_ARG, "rg_adam_step: null tensor table");
+    rg::AdamBatch b{};
+    b.count = count;
+    int64_t items = 0;
+    for (int k = 0; k < count; ++k) {
+        const rg_adam_tensor& t = tensors[k];
+        if (t.n < 0) return fail_free(RG_ERR_ARG, "rg_adam_step: negative size");
+        if (t.n > 0 && (!t.param || !t.grad || !t.exp_avg || !t.exp_avg_sq))
+            return fail_free(RG_ERR_ARG, "rg_adam_step: null array");
+        if (t.step < 1) return fail_free(RG_ERR_ARG, "rg_adam_step: step must be >= 1 (count after the update)");
+        if (!(t.beta1 >= 0.0 && t.beta1 < 1.0 && t.beta2 >= 0.0 && t.beta2 < 1.0 && t.eps >= 0.0 && t.lr >= 0.0))
+            return fail_free(RG_ERR_ARG, "rg_adam_step: lr, eps >= 0 and betas in [0, 1) required");
+        b.t[k].param = t.param;
+        b.t[k].grad = t.grad;
+        b.t[k].exp_avg = t.exp_avg;
+        b.t[k].exp_avg_sq = t.exp_avg_sq;
+        b.t[k].n = t.n;
+        auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+        b.t[k].aligned = al(t.param) && al(t.grad) && al(t.exp_avg) && al(t.exp_avg_sq);
+        const double bc1 = 1.0 - std::pow(t.beta1, (double)t.step);
+        const double bc2 = 1.0 - std::pow(t.beta2, (double)t.step);
+        b.s[k].w1 = (float)(1.0 - t.beta1);
+        b.s[k].b2 = (float)t.beta2;
+        b.s[k].w2 = (float)(1.0 - t.beta2);
+        b.s[k].bc2_sqrt = (float)std::sqrt(bc2);
+        b.s[k].eps = (float)t.eps;
+        b.s[k].neg_step = (float)(-(t.lr / bc1));
+        b.item_start[k] = items;
+        items += (t.n + 3) / 4;
+    }
+    b.item_start[count] = items;
+    const hipError_t e = rg::launch_adam(b, items, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_adam_step: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int rg_project_params(const rg_projection* pr, void* stream) {
+    if (!pr) return fail_free(RG_ERR_ARG, "rg_project_params: null argument");
+    if (pr->n < 0 || pr->num_sg < 0) return fail_free(RG_ERR_ARG, "rg_project_params: negative size");
+    if (pr->normalize_lobe_axis && pr->num_sg > 0 && pr->n > 0 && !pr->lobe_axis)
+        return fail_free(RG_ERR_ARG, "rg_project_params: null lobe_axis with num_sg > 0");
+    const hipError_t e = rg::launch_project(*pr, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_project_params: ") + hipGetErrorString(e));
     return RG_OK;
 }
 

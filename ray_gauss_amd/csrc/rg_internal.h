@@ -98,6 +98,35 @@ hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b,
 hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s);
 hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s);
 
+// ---- launchers (optim.hip) ---------------------------------------------------------------------
+constexpr int kMaxAdamTensors = 16;
+// Per-tensor scalars of one Adam step, rounded to float once on the host.
+struct AdamScalars {
+    float w1;        // 1 - beta1
+    float b2;        // beta2
+    float w2;        // 1 - beta2
+    float bc2_sqrt;  // sqrt(1 - beta2^step)
+    float eps;
+    float neg_step;  // -lr / (1 - beta1^step)
+};
+struct AdamTensor {
+    float* param;
+    const float* grad;
+    float* exp_avg;
+    float* exp_avg_sq;
+    int64_t n;
+    int aligned;  // all four arrays 16-B aligned
+};
+// Work items are float4 groups: tensor k owns items [item_start[k], item_start[k+1]).
+struct AdamBatch {
+    AdamTensor t[kMaxAdamTensors];
+    AdamScalars s[kMaxAdamTensors];
+    int64_t item_start[kMaxAdamTensors + 1];
+    int count;
+};
+hipError_t launch_adam(const AdamBatch& b, int64_t total_items, hipStream_t s);
+hipError_t launch_project(const rg_projection& pr, hipStream_t s);
+
 // ---- launchers (trace.hip) --------------------------------------------------------------------
 struct RayJob {
     int64_t num_rays;

@@ -113,3 +113,26 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built")
+def test_optimiser_entry_points_reject_bad_arguments_on_the_host():
+    """Argument checks of rg_adam_step / rg_project_params run before any device work."""
+    from ray_gauss_amd import _lib
+
+    lib = _lib.load()
+    table = (_lib.AdamTensor * 17)()
+    assert lib.rg_adam_step(table, 17, None) != 0
+    assert b"16" in lib.rg_last_error(None)
+    one = (_lib.AdamTensor * 1)()
+    one[0].n = 4
+    one[0].step = 1
+    assert lib.rg_adam_step(one, 1, None) != 0  # null arrays
+    assert b"null" in lib.rg_last_error(None)
+    one[0].n = 0
+    one[0].step = 0
+    assert lib.rg_adam_step(one, 1, None) != 0  # step must count the update
+    assert lib.rg_project_params(None, None) != 0
+    pr = _lib.Projection()
+    pr.n = -1
+    assert lib.rg_project_params(ctypes.byref(pr), None) != 0
