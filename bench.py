@@ -110,6 +110,97 @@ def cpu_baseline(wl, threads=None):
                       f"CPU BVH build, {dt:.1f} s"}
 
 
+# learning rates of the full-step benchmark (the timing does not depend on them)
+FULL_STEP_LR = {"positions": 1.6e-4, "rgb": 2.5e-3, "sh_rest": 1.25e-4, "log_scales": 5e-3, "quats": 1e-3,
+                "densities": 5e-2, "sg": 2.5e-3, "sharpness_raw": 1e-2, "lobe_axis": 1e-3}
+
+
+def full_step(args):
+    """The complete training step of the 3M-Gaussian workload: per step mu += N(0, 1e-3) and
+    log s += N(0, 1e-2), the reference's activations (exp scales, normalised quaternions and lobe
+    axes, softplus sharpness, densities as they are: classes/point_cloud.py:689-739) with their
+    autograd, the graph-captured refit + render + loss + backward, one Adam launch over the nine
+    parameter tensors and the post-step projections (scripts/train.py:121-124, 168-183)."""
+    import numpy as np
+    import torch
+
+    from ray_gauss_amd import scenes
+    from ray_gauss_amd.batch import BatchedTrainer, make_ray_batch
+    from ray_gauss_amd.caster import RayCaster, TuneConfig
+    from ray_gauss_amd.optim import FusedAdam, project_params, step_all
+
+    dev = torch.device("cuda:0")
+    wl = scenes.workload(args.workload)
+    sc = wl.scene
+    cams = [c.launch_record() for c in wl.cameras]
+    f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)  # noqa: E731
+    raw = {"positions": f32(sc.positions), "rgb": f32(sc.sh[:, :, 0]), "sh_rest": f32(sc.sh[:, :, 1:]),
+           "log_scales": f32(np.log(sc.scales)), "quats": f32(sc.quats), "densities": f32(sc.densities),
+           "sg": f32(sc.sg), "sharpness_raw": f32(np.log(np.expm1(sc.sharpness))), "lobe_axis": f32(sc.lobe_axis)}
+    for t in raw.values():
+        t.requires_grad_(True)
+
+    def activate():
+        return {"positions": raw["positions"], "scales": torch.exp(raw["log_scales"]),
+                "quats": raw["quats"] / torch.norm(raw["quats"], dim=1)[:, None], "densities": raw["densities"],
+                "sh": torch.cat([raw["rgb"][:, :, None], raw["sh_rest"]], dim=2), "sg": raw["sg"],
+                "sharpness": torch.nn.functional.softplus(raw["sharpness_raw"]),
+                "lobe_axis": raw["lobe_axis"] / torch.norm(raw["lobe_axis"], dim=2)[:, :, None]}
+
+    with torch.no_grad():
+        buf = {k: v.detach().clone().contiguous() for k, v in activate().items()}
+    rv, rp, _ = make_ray_batch(cams, wl.pixels, dev)
+    R = int(rv.shape[0])
+    target = torch.rand((R, 3), generator=torch.Generator(device=dev).manual_seed(0), device=dev)
+    tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet)
+    trainer = BatchedTrainer(RayCaster(wl.params, tune), buf, cams, rv, rp, target, loss_kind=0,
+                             use_graph=not args.no_graph)
+    opts = [FusedAdam([raw[k]], lr=FULL_STEP_LR[k]) for k in raw]
+    gen = torch.Generator(device=dev).manual_seed(1)
+    filter_3d = torch.full((sc.n, 1), math.log(1e-4), device=dev)
+
+    def step():
+        with torch.no_grad():
+            raw["positions"].add_(torch.randn(raw["positions"].shape, generator=gen, device=dev), alpha=1e-3)
+            raw["log_scales"].add_(torch.randn(raw["log_scales"].shape, generator=gen, device=dev), alpha=1e-2)
+        act = activate()
+        with torch.no_grad():
+            for k, v in act.items():
+                buf[k].copy_(v)
+        g = trainer.step()
+        names = [k for k in act if g.get(k) is not None]
+        torch.autograd.backward([act[k] for k in names], [g[k] for k in names])
+        step_all(opts)
+        project_params(densities=raw["densities"].data, log_scales=raw["log_scales"].data, filter_3d=filter_3d,
+                       max_scale=float("inf"), quaternions=raw["quats"].data, lobe_axis=raw["lobe_axis"].data)
+        for o in opts:
+            o.zero_grad(set_to_none=True)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    peak_used = total - free0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        free, _ = torch.cuda.mem_get_info()
+        peak_used = max(peak_used, total - free)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    n_params = sum(t.numel() for t in raw.values())
+    print(json.dumps({
+        "metric": "full train step (refit + fwd + bwd + Adam + projections), rays/s",
+        "value": R / dt, "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "gaussians": sc.n, "views": len(cams), "rays_per_step": R,
+                   "sg_lobes": sc.num_sg, "parameters": n_params},
+        "device_memory_used_peak_GB": peak_used / 1e9,
+        "adam_bytes_per_step": 28 * n_params,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +215,15 @@ def main():
     ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
+    ap.add_argument("--full-step", action="store_true",
+                    help="time the complete training step incl. Adam (use with --workload unbounded3m_4views)")
     args = ap.parse_args()
+    if args.full_step:
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU", "n_gpus": args.gpus}))
+            sys.exit(2)
+        full_step(args)
+        return
     if args.calibrate_copy:
         import torch
 
