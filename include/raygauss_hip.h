@@ -232,6 +232,22 @@ typedef struct rg_projection {
 } rg_projection;
 int rg_project_params(const rg_projection* pr, void* stream);
 
+/* ---- photometric loss (handle-free) -------------------------------------------------------- */
+
+/* Device workspace bytes rg_loss_l1_ssim needs for a width x height image. */
+int64_t rg_loss_workspace_bytes(int32_t width, int32_t height);
+
+/* L = (1 - lambda) * mean|x - y| + lambda * (1 - SSIM(x, y)), SSIM with the 11x11 Gaussian window
+ * (sigma 1.5), zero padding, C1 = 0.01^2, C2 = 0.03^2 (utils/loss_utils.py:17-64; train.py:110-112
+ * uses lambda = 0.2), where x is the rendered image after the supersampling mean
+ * (utils/general_utils.py:489-502).  ray_colors: (height*ss_y, width*ss_x, 3) row-major rays;
+ * target: (3, height, width).  out_loss: device float[1]; out_parts (optional): device float[2] =
+ * (L1, SSIM); dL_dC (optional): gradient w.r.t. ray_colors, same layout.  No reference counterpart
+ * as one routine: it fuses the caller's reduce_supersampling + l1_loss + ssim + their autograd. */
+int rg_loss_l1_ssim(const float* ray_colors, const float* target, int32_t width, int32_t height, int32_t ss_x,
+                    int32_t ss_y, float lambda_ssim, float* out_loss, float* out_parts, float* dL_dC,
+                    void* workspace, int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

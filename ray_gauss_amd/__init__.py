@@ -5,6 +5,7 @@ Public surface:
   optix_utils                               -- host-glue names used by the training loop
   RayCaster, TuneConfig                     -- direct handle on the HIP library
   FusedAdam, step_all, project_params       -- optimiser step + post-step projections (optim.py)
+  l1_ssim_loss                              -- the training loss and its gradient (loss.py)
   scenes, cameras                           -- synthetic workloads and pinhole cameras
 """
 
@@ -14,7 +15,7 @@ from .scenes import RenderParams  # noqa: F401
 
 _LAZY = {"RenderOptixSettings": "render", "RenderOptixFunction": "render", "render": "render",
          "RayCaster": "caster", "TuneConfig": "caster", "FusedAdam": "optim", "step_all": "optim",
-         "project_params": "optim"}
+         "project_params": "optim", "l1_ssim_loss": "loss"}
 
 
 def __getattr__(name):

@@ -549,4 +549,34 @@ int rg_project_params(const rg_projection* pr, void* stream) {
     return RG_OK;
 }
 
+int64_t rg_loss_workspace_bytes(int32_t width, int32_t height) {
+    if (width <= 0 || height <= 0) return 0;
+    return rg::loss_workspace_bytes(width, height);
+}
+
+int rg_loss_l1_ssim(const float* ray_colors, const float* target, int32_t width, int32_t height, int32_t ss_x,
+                    int32_t ss_y, float lambda_ssim, float* out_loss, float* out_parts, float* dL_dC,
+                    void* workspace, int64_t workspace_bytes, void* stream) {
+    if (!ray_colors || !target || !out_loss || !workspace)
+        return fail_free(RG_ERR_ARG, "rg_loss_l1_ssim: null argument");
+    if (width <= 0 || height <= 0 || ss_x <= 0 || ss_y <= 0)
+        return fail_free(RG_ERR_ARG, "rg_loss_l1_ssim: sizes must be positive");
+    if (workspace_bytes < rg::loss_workspace_bytes(width, height))
+        return fail_free(RG_ERR_ARG, "rg_loss_l1_ssim: workspace smaller than rg_loss_workspace_bytes()");
+    // the window as the reference builds it: float Gaussian taps normalised by their float sum
+    float w[11];
+    float sum = 0.0f;
+    for (int i = 0; i < 11; ++i) {
+        const double d = (double)(i - 5);
+        w[i] = (float)std::exp(-(d * d) / (2.0 * 1.5 * 1.5));
+        sum += w[i];
+    }
+    for (int i = 0; i < 11; ++i) w[i] /= sum;
+    const hipError_t e = rg::launch_loss_l1_ssim(ray_colors, target, width, height, ss_x, ss_y, w, lambda_ssim,
+                                                 out_loss, out_parts, dL_dC, workspace,
+                                                 static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_loss_l1_ssim: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
 }  // extern "C"

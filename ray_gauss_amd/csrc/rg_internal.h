@@ -127,6 +127,12 @@ struct AdamBatch {
 hipError_t launch_adam(const AdamBatch& b, int64_t total_items, hipStream_t s);
 hipError_t launch_project(const rg_projection& pr, hipStream_t s);
 
+// ---- launchers (loss.hip) ----------------------------------------------------------------------
+int64_t loss_workspace_bytes(int W, int H);
+hipError_t launch_loss_l1_ssim(const float* rays, const float* target, int W, int H, int fx, int fy,
+                               const float* window11, float lambda, float* loss, float* parts, float* dl_rays,
+                               void* ws, hipStream_t s);
+
 // ---- launchers (trace.hip) --------------------------------------------------------------------
 struct RayJob {
     int64_t num_rays;

@@ -21,7 +21,7 @@ LIB = os.path.join(ROOT, "ray_gauss_amd", "_build", "libraygauss_hip.so")
 
 def header_functions():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(rg_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|const char\*)\s+(rg_\w+)\(", txt, re.M)))
 
 
 def test_header_declares_the_abi():
