@@ -49,6 +49,9 @@ typedef struct rg_config {
     int32_t max_hits_per_slab; /* >0: count slabs whose hit set exceeds it (max_prim_slice)      */
     int32_t packet_traversal;  /* 1: the lanes of a wavefront walk the LBVH together (one node
                                   sequence, per-lane box tests); 0: per-lane walks                */
+    int32_t jitter;            /* RenderOptixSettings.jitter: sub-pixel jitter (gaussians_aabb.cu:409) */
+    int32_t rnd_sample;        /* RenderOptixSettings.rnd_sample: 0 grid, 1 random, 2 mid-sample     */
+    uint32_t iteration;        /* RenderOptixSettings.iteration: seeds tea<4>(iteration, pixel)       */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

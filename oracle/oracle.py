@@ -28,7 +28,8 @@ class Camera(C.Structure):
 class Params(C.Structure):
     _fields_ = [("dt", C.c_float), ("spb", C.c_int32), ("t_eps", C.c_float), ("sigma_thr", C.c_float),
                 ("t_near", C.c_float), ("t_far", C.c_float), ("white_bg", C.c_int32),
-                ("exact_sh_grad", C.c_int32)]
+                ("exact_sh_grad", C.c_int32), ("jitter", C.c_int32), ("rnd_sample", C.c_int32),
+                ("iteration", C.c_uint32)]
 
 
 class SceneS(C.Structure):
@@ -91,6 +92,9 @@ def params_struct(prm, exact_sh_grad=False) -> Params:
     p.t_far = float(np.float32(prm.t_far))
     p.white_bg = int(bool(prm.white_background))
     p.exact_sh_grad = int(bool(exact_sh_grad))
+    p.jitter = int(getattr(prm, "jitter", 0))
+    p.rnd_sample = int(getattr(prm, "rnd_sample", 0))
+    p.iteration = int(getattr(prm, "iteration", 0)) & 0xFFFFFFFF
     return p
 
 
@@ -107,6 +111,38 @@ class _SceneHolder:
         for k, a in zip(("pos", "scale", "quat", "dens", "sh", "sg", "lam", "axis"), self.arrays):
             setattr(s, k, a.ctypes.data)
         self.s = s
+
+
+def rays_rng(cam_rec, pixels=None, jitter=0, iteration=0):
+    """Rays with the reference's sub-pixel jitter (generator seeded per pixel)."""
+    cs = camera_struct(cam_rec)
+    n = cam_rec["width"] * cam_rec["height"] if pixels is None else len(pixels)
+    px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+    o = np.empty((n, 3), np.float32)
+    d = np.empty((n, 3), np.float32)
+    lib().orc_rays_rng(C.byref(cs), C.c_int64(n), _p(px), C.c_int32(int(jitter)), C.c_uint32(int(iteration)),
+                       _p(o), _p(d))
+    return o, d
+
+
+def tea(v0, v1, cycles, key):
+    k = np.ascontiguousarray(key, np.uint32)
+    f = lib().orc_tea
+    f.restype = C.c_uint32
+    return int(f(C.c_uint32(v0), C.c_uint32(v1), C.c_int32(cycles), _p(k)))
+
+
+def tea4(v0, v1):
+    f = lib().orc_tea4
+    f.restype = C.c_uint32
+    return int(f(C.c_uint32(v0), C.c_uint32(v1)))
+
+
+def lcg(seed, n):
+    ints = np.empty(n, np.uint32)
+    floats = np.empty(n, np.float32)
+    lib().orc_lcg(C.c_uint32(seed), C.c_int32(n), _p(ints), _p(floats))
+    return ints, floats
 
 
 def rays(cam_rec, pixels=None):

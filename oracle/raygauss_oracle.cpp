@@ -25,6 +25,13 @@
 //   forward_sh / backward_sh    utils/sh_utils.cu:25-119 / :121-315
 //   compute_ellipsoids_bbox     optix_raycasting/optix_utils.py:54-66
 //   depth (viewer only)         optix_raycasting/cuda_gui/gaussians_aabb.cu:397-417
+//   tea<4>, lcg, rnd            optix_raycasting/cuda_train/forward/gaussians_aabb.h:63-94
+//
+// Random numbers (jitter, rnd_sample): a ray's generator is seeded with tea<4>(iteration,
+// pixel) (:526); jitter draws two numbers in computeRay (:413-414); with rnd_sample = 1 every
+// slab that has hits draws one number per sample, shared by all its hits, and the generator
+// then stands after those draws (:440-496; backward :573-611, :768); slabs without hits draw
+// nothing.
 //
 // Runtime parameters replace the reference's compile-time constants (gaussians_aabb.h:2-5):
 // dt (DT), samples per slab (BUFFER_SIZE), t_eps (TRANSMITTANCE_EPSILON), sigma_thr
@@ -70,6 +77,9 @@ struct orc_params {
     float t_far;
     int32_t white_bg;
     int32_t exact_sh_grad;  // 0: reference backward_sh (utils/sh_utils.cu:254) ; 1: exact derivative
+    int32_t jitter;         // 1: sub-pixel jitter of the ray (forward/gaussians_aabb.cu:409-418)
+    int32_t rnd_sample;     // 0: t_i = tb + i dt; 1: + u_i dt, u_i random; 2: + dt/2 (:473-482)
+    uint32_t iteration;     // seeds the per-ray generator with tea<4>(iteration, pixel) (:526)
 };
 
 struct orc_scene {
@@ -99,13 +109,43 @@ struct Ray32 {
     float d[3];
 };
 
-// computeRay (forward/gaussians_aabb.cu:405-434), jitter off.
-Ray32 make_ray32(const orc_camera& c, int64_t pixel) {
+// The reference's generator (forward/gaussians_aabb.h:63-94): TEA with 2N rounds of the
+// Wheeler-Needham cipher and the fixed key (0xa341316c, 0xc8013ea4, 0xad90777d, 0x7e95761e)
+// seeds; a 32-bit LCG (1664525, 1013904223) steps; rnd = low 24 bits / 2^24.
+uint32_t tea_keyed(uint32_t v0, uint32_t v1, int cycles, const uint32_t k[4]) {
+    uint32_t s0 = 0;
+    for (int n = 0; n < cycles; ++n) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + k[0]) ^ (v1 + s0) ^ ((v1 >> 5) + k[1]);
+        v1 += ((v0 << 4) + k[2]) ^ (v0 + s0) ^ ((v0 >> 5) + k[3]);
+    }
+    return v0;
+}
+uint32_t tea4(uint32_t v0, uint32_t v1) {
+    static const uint32_t k[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+    return tea_keyed(v0, v1, 4, k);
+}
+inline uint32_t lcg(uint32_t& s) {
+    s = 1664525u * s + 1013904223u;
+    return s & 0x00FFFFFFu;
+}
+inline float rnd(uint32_t& s) { return (float)lcg(s) / (float)0x01000000; }
+
+// computeRay (forward/gaussians_aabb.cu:405-434); with jitter the two draws advance seed.
+Ray32 make_ray32(const orc_camera& c, int64_t pixel, int jitter = 0, uint32_t* seed = nullptr) {
     Ray32 r;
     const int64_t ix = pixel % c.width;
     const int64_t iy = pixel / c.width;
-    const float fx = ((float)ix + 0.5f) / (float)c.width;
-    const float fy = ((float)iy + 0.5f) / (float)c.height;
+    float fx, fy;
+    if (jitter) {
+        const float jx = rnd(*seed) * 0.333333f;
+        const float jy = rnd(*seed) * 0.333333f;
+        fx = (((float)ix + 0.333333f) + jx) / (float)c.width;
+        fy = (((float)iy + 0.333333f) + jy) / (float)c.height;
+    } else {
+        fx = ((float)ix + 0.5f) / (float)c.width;
+        fy = ((float)iy + 0.5f) / (float)c.height;
+    }
     const float px = 2.0f * fx - 1.0f;
     const float py = 2.0f * fy - 1.0f;
     const float dx = px * c.tanx;
@@ -142,6 +182,27 @@ void scene_interval32(const Ray32& r, const float* bmin, const float* bmax, floa
 inline float sample_t32(float tb, int i, float dt) {
     const float off = (float)i * dt;
     return tb + off;
+}
+
+// Sample positions of one slab (FP32, :473-482): t_i = (tb + i dt) [+ u_i dt | + dt/2]; with
+// rnd_sample = 1, u_i is the (i+1)-th draw from the slab's generator state `seed`.
+struct SlabSamples {
+    float t[16];
+    SlabSamples(const orc_params& p, float tb, uint32_t seed) {
+        for (int i = 0; i < p.spb; ++i) {
+            const float base = sample_t32(tb, i, p.dt);
+            if (p.rnd_sample == 1) t[i] = base + rnd(seed) * p.dt;
+            else if (p.rnd_sample == 2) t[i] = base + 0.5f * p.dt;
+            else t[i] = base;
+        }
+    }
+};
+
+// The generator after a slab with hits (spb draws when rnd_sample = 1).
+inline uint32_t after_slab(const orc_params& p, uint32_t seed) {
+    if (p.rnd_sample == 1)
+        for (int i = 0; i < p.spb; ++i) lcg(seed);
+    return seed;
 }
 
 // ---------------------------------------------------------------- FP64 geometry -------------
@@ -585,14 +646,14 @@ inline double gauss_w(const Geo& G, const double x[3], double y[3]) {
 
 // computeBufferForward (forward/gaussians_aabb.cu:437-498) for one slab, FP64.
 void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const double* rgb,
-                     const std::vector<Hit>& H, const std::vector<int>& cur, const Ray32& r, float tb,
+                     const std::vector<Hit>& H, const std::vector<int>& cur, const Ray32& r, const SlabSamples& ts,
                      SampleBuf& buf, int* fragile, int64_t* nc = nullptr, const std::vector<int>* near = nullptr) {
     const double thr = p.sigma_thr;
     if (near)
         for (int a : *near) {
             const Geo& G = geo[H[a].g];
             for (int i = 0; i < p.spb; ++i) {
-                const double t = sample_t32(tb, i, p.dt);
+                const double t = ts.t[i];
                 double x[3], y[3];
                 for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
                 if (G.sigma * gauss_w(G, x, y) >= thr * (1.0 - kEpsW)) *fragile = 1;
@@ -601,7 +662,7 @@ void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const dou
     for (int a : cur) {
         const Geo& G = geo[H[a].g];
         for (int i = 0; i < p.spb; ++i) {
-            const double t = sample_t32(tb, i, p.dt);
+            const double t = ts.t[i];
             double x[3], y[3];
             for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
             const double w = gauss_w(G, x, y);
@@ -622,11 +683,13 @@ struct RayCtx {
     Ray32 r;
     double o[3], d[3];
     float tenter, texit;
+    uint32_t seed;  // generator state after ray generation
 };
 
 RayCtx make_ctx(const orc_params& p, const orc_camera& cam, int64_t pixel, const float* bmin, const float* bmax) {
     RayCtx c;
-    c.r = make_ray32(cam, pixel);
+    c.seed = tea4(p.iteration, (uint32_t)pixel);
+    c.r = make_ray32(cam, pixel, p.jitter, &c.seed);
     for (int k = 0; k < 3; ++k) {
         c.o[k] = c.r.o[k];
         c.d[k] = c.r.d[k];
@@ -655,6 +718,7 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
     const double dt = p.dt;
     const double eps = p.t_eps;
     float tb = c.tenter;
+    uint32_t seed = c.seed;
     std::vector<char> seen(hits.size(), 0);
     while (tb < c.texit && T > eps) {
         if (near_rel(T, eps, 1e-3)) out.fragile = 1;
@@ -663,10 +727,11 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
         out.slabs++;
         if (tmax > c.tenter) {
             W.step(tmin, tmax, &out.fragile);
+            const SlabSamples ts(p, tb, seed);
             if (W.cur.empty() && !W.near.empty()) {
                 buf.reset(p.spb);
                 const std::vector<int> none;
-                accumulate_slab(p, geo, rgb, hits, none, c.r, tb, buf, &out.fragile, nullptr, &W.near);
+                accumulate_slab(p, geo, rgb, hits, none, c.r, ts, buf, &out.fragile, nullptr, &W.near);
             }
             if (!W.cur.empty()) {
                 out.K += (int64_t)W.cur.size();
@@ -677,7 +742,7 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
                         out.P++;
                     }
                 buf.reset(p.spb);
-                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, tb, buf, &out.fragile, &out.nc, &W.near);
+                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, ts, buf, &out.fragile, &out.nc, &W.near);
                 for (int i = 0; i < p.spb; ++i) {
                     const double s = buf.sig[i];
                     const double alpha = 1.0 - std::exp(-s * dt);
@@ -685,9 +750,10 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
                     if (s > 0.0)
                         for (int k = 0; k < 3; ++k) ch[k] /= s;
                     for (int k = 0; k < 3; ++k) out.C[k] += T * alpha * ch[k];
-                    out.depth += T * alpha * (double)sample_t32(tb, i, p.dt);
+                    out.depth += T * alpha * (double)ts.t[i];
                     T *= 1.0 - alpha;
                 }
+                seed = after_slab(p, seed);
             } else if (W.exhausted()) {
                 // nothing left on this ray: the remaining slabs are empty; skip them (T unchanged)
                 float t = tb;
@@ -717,6 +783,7 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
     }
     const float slab = p.dt * (float)p.spb;
     float tb = c.tenter;
+    uint32_t seed = c.seed;
     std::vector<Hit> hits;
     std::vector<int> cur;
     SampleBuf buf;
@@ -736,7 +803,8 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
                 out.K += (int64_t)hits.size();
                 buf.reset(p.spb);
                 int dummy = 0;
-                accumulate_slab(p, geo, rgb, hits, cur, c.r, tb, buf, &dummy);
+                const SlabSamples ts(p, tb, seed);
+                accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &dummy);
                 for (int i = 0; i < p.spb; ++i) {
                     const double s = buf.sig[i];
                     const double alpha = 1.0 - std::exp(-s * (double)p.dt);
@@ -744,9 +812,10 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
                     if (s > 0.0)
                         for (int k = 0; k < 3; ++k) ch[k] /= s;
                     for (int k = 0; k < 3; ++k) out.C[k] += T * alpha * ch[k];
-                    out.depth += T * alpha * (double)sample_t32(tb, i, p.dt);
+                    out.depth += T * alpha * (double)ts.t[i];
                     T *= 1.0 - alpha;
                 }
+                seed = after_slab(p, seed);
             }
         }
         tb = tb + slab;
@@ -778,6 +847,7 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
     double T = 1.0;
     double Crem[3] = {Cfwd[0], Cfwd[1], Cfwd[2]};
     float tb = c.tenter;
+    uint32_t seed = c.seed;
     const int spb = p.spb;
     std::vector<double> alpha(spb), chat(3 * spb);
     while (tb < c.texit && T > (double)p.t_eps) {
@@ -787,7 +857,9 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
             W.step(tmin, tmax, &fragile);
             if (!W.cur.empty()) {
                 buf.reset(spb);
-                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, tb, buf, &fragile);
+                const SlabSamples ts(p, tb, seed);
+                seed = after_slab(p, seed);
+                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, ts, buf, &fragile);
                 for (int i = 0; i < spb; ++i) {
                     const double s = buf.sig[i];
                     double ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
@@ -811,7 +883,7 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
                         const double cd = buf.sig[i];
                         const double* cc = &chat[3 * i];
                         const double al = alpha[i];
-                        const double t = sample_t32(tb, i, p.dt);
+                        const double t = ts.t[i];
                         double x[3], y[3];
                         for (int k = 0; k < 3; ++k) x[k] = (double)c.r.o[k] + t * (double)c.r.d[k];
                         const double w = gauss_w(G, x, y);
@@ -901,6 +973,34 @@ void orc_rays(const orc_camera* cam, int64_t nrays, const int32_t* pixels, float
             o[3 * r + k] = R.o[k];
             d[3 * r + k] = R.d[k];
         }
+    }
+}
+
+// Ray generation with the reference's jitter (generator seeded with tea<4>(iteration, pixel)).
+void orc_rays_rng(const orc_camera* cam, int64_t nrays, const int32_t* pixels, int32_t jitter, uint32_t iteration,
+                  float* o, float* d) {
+    for (int64_t r = 0; r < nrays; ++r) {
+        const int64_t px = pixels ? pixels[r] : r;
+        uint32_t seed = tea4(iteration, (uint32_t)px);
+        const Ray32 R = make_ray32(*cam, px, jitter, &seed);
+        for (int k = 0; k < 3; ++k) {
+            o[3 * r + k] = R.o[k];
+            d[3 * r + k] = R.d[k];
+        }
+    }
+}
+
+// The generator's pieces, for pinning: TEA with an explicit key and cycle count, and the first
+// n LCG outputs (24-bit integers and rnd floats) from a seed.
+uint32_t orc_tea(uint32_t v0, uint32_t v1, int32_t cycles, const uint32_t* key) {
+    return tea_keyed(v0, v1, cycles, key);
+}
+uint32_t orc_tea4(uint32_t v0, uint32_t v1) { return tea4(v0, v1); }
+void orc_lcg(uint32_t seed, int32_t n, uint32_t* ints, float* floats) {
+    uint32_t s = seed, t = seed;
+    for (int i = 0; i < n; ++i) {
+        ints[i] = lcg(s);
+        floats[i] = rnd(t);
     }
 }
 

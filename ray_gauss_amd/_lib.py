@@ -19,7 +19,8 @@ class Config(C.Structure):
     _fields_ = [("dt", C.c_float), ("samples_per_slab", C.c_int32), ("t_eps", C.c_float), ("sigma_thr", C.c_float),
                 ("t_near", C.c_float), ("t_far", C.c_float), ("white_background", C.c_int32),
                 ("exact_sh_grad", C.c_int32), ("hit_depth", C.c_int32), ("seg_slabs", C.c_int32),
-                ("max_hits_per_slab", C.c_int32), ("packet_traversal", C.c_int32)]
+                ("max_hits_per_slab", C.c_int32), ("packet_traversal", C.c_int32),
+                ("jitter", C.c_int32), ("rnd_sample", C.c_int32), ("iteration", C.c_uint32)]
 
 
 class CameraS(C.Structure):

@@ -103,6 +103,9 @@ class RayCaster:
         c.seg_slabs = int(self.tune.seg_slabs)
         c.max_hits_per_slab = int(self.max_hits_per_slab)
         c.packet_traversal = int(bool(self.tune.packet))
+        c.jitter = int(getattr(p, "jitter", 0))
+        c.rnd_sample = int(getattr(p, "rnd_sample", 0))
+        c.iteration = int(getattr(p, "iteration", 0) or 0) & 0xFFFFFFFF
         return c
 
     def set_tune(self, tune: TuneConfig):

@@ -90,6 +90,8 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->hit_depth != 8 && c->hit_depth != 16 && c->hit_depth != 32)
         return fail(h, RG_ERR_ARG, "hit_depth must be 8, 16 or 32");
     if (c->seg_slabs < 1 || c->seg_slabs > 4096) return fail(h, RG_ERR_ARG, "seg_slabs in 1..4096");
+    if (c->jitter != 0 && c->jitter != 1) return fail(h, RG_ERR_ARG, "jitter must be 0 or 1");
+    if (c->rnd_sample < 0 || c->rnd_sample > 2) return fail(h, RG_ERR_ARG, "rnd_sample must be 0, 1 or 2");
     return RG_OK;
 }
 
@@ -167,6 +169,9 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.max_hits = h->cfg.max_hits_per_slab;
     p.seg_slabs = h->cfg.seg_slabs;
     p.packet = h->cfg.packet_traversal;
+    p.jitter = h->cfg.jitter;
+    p.rnd_sample = h->cfg.rnd_sample;
+    p.iteration = h->cfg.iteration;
     return p;
 }
 

@@ -18,15 +18,50 @@ struct Ray {
     float tenter, texit;
 };
 
-// computeRay (reference cuda_train/forward/gaussians_aabb.cu:405-434, jitter off) followed by
-// the scene-box slab test (:530-541) clipped to [t_near, t_far].  Exact FP32.
-__device__ __forceinline__ Ray make_ray(const rg_camera& c, int pixel, const float* box, float t_near, float t_far) {
+// The reference's random generator (forward/gaussians_aabb.h:63-94): tea<4> (four cycles of the
+// TEA cipher with a fixed key) seeds a ray from (iteration, pixel); a 32-bit LCG steps it and a
+// draw is its low 24 bits / 2^24.
+__device__ __forceinline__ unsigned int tea4(unsigned int v0, unsigned int v1) {
+    unsigned int s0 = 0;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+__device__ __forceinline__ float rnd01(unsigned int& st) {
+    st = 1664525u * st + 1013904223u;
+    return (float)(st & 0x00FFFFFFu) / 16777216.0f;
+}
+
+// Generator state after n draws.
+__device__ __forceinline__ unsigned int skip_draws(unsigned int st, int n) {
+    for (int i = 0; i < n; ++i) st = 1664525u * st + 1013904223u;
+    return st;
+}
+
+// computeRay (reference cuda_train/forward/gaussians_aabb.cu:405-434) followed by the scene-box
+// slab test (:530-541) clipped to [t_near, t_far].  Exact FP32.  With jitter the pixel position
+// is x + 1/3 + u/3 (two draws from the ray's generator, :409-418), else x + 1/2.
+__device__ __forceinline__ Ray make_ray(const rg_camera& c, int pixel, const float* box, float t_near, float t_far,
+                                        int jitter = 0, unsigned int* seed = nullptr) {
 #pragma clang fp contract(off)
     Ray r;
     const int ix = pixel % c.width;
     const int iy = pixel / c.width;
-    const float fx = ((float)ix + 0.5f) / (float)c.width;
-    const float fy = ((float)iy + 0.5f) / (float)c.height;
+    float fx, fy;
+    if (jitter) {
+        const float jx = rnd01(*seed) * 0.333333f;
+        const float jy = rnd01(*seed) * 0.333333f;
+        fx = (((float)ix + 0.333333f) + jx) / (float)c.width;
+        fy = (((float)iy + 0.333333f) + jy) / (float)c.height;
+    } else {
+        fx = ((float)ix + 0.5f) / (float)c.width;
+        fy = ((float)iy + 0.5f) / (float)c.height;
+    }
     const float px = 2.0f * fx - 1.0f;
     const float py = 2.0f * fy - 1.0f;
     const float ddx = px * c.tanx;
@@ -59,11 +94,16 @@ __device__ __forceinline__ float add_exact(float a, float b) {
     return a + b;
 }
 
-// Exact FP32: t of sample i in the slab starting at tb (reference :474).
-__device__ __forceinline__ float sample_t(float tb, int i, float dt) {
+// Exact FP32: t of sample i in the slab starting at tb (reference :474), with the rnd_sample
+// offsets of :473-482 -- mode 1: + u dt, u the next draw of st (each hit of a slab restarts from
+// the slab's state); mode 2: + dt/2.
+__device__ __forceinline__ float sample_t(float tb, int i, float dt, int mode, unsigned int& st) {
 #pragma clang fp contract(off)
     const float off = (float)i * dt;
-    return tb + off;
+    const float base = tb + off;
+    if (mode == 1) return base + rnd01(st) * dt;
+    if (mode == 2) return base + 0.5f * dt;
+    return base;
 }
 
 // Per (ray, Gaussian) result of the bounding-ellipsoid test.

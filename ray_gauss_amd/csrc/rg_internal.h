@@ -48,6 +48,9 @@ struct DevParams {
     int max_hits;    // >0: count slabs with more hits (reference max_prim_slice)
     int seg_slabs;   // initial segment length
     int packet;      // 1: wave-uniform packet traversal, 0: per-lane traversal
+    int jitter;      // sub-pixel jitter of the rays
+    int rnd_sample;  // sample offsets: 0 none, 1 random, 2 half a sample
+    unsigned int iteration;  // seeds each ray's generator: tea4(iteration, pixel)
 };
 
 // Gradient accumulators (device-scope atomic adds; the finish kernel reads and clears them).

@@ -125,13 +125,14 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ geo, int s, 
 
 // Accumulate one hit's contribution to the slab's sample buffers (computeBufferForward,
 // forward/gaussians_aabb.cu:437-498).
-template <int SPB>
-__device__ __forceinline__ void accum_hit(const Isect& I, float sigma, float4 col, float tb, const DevParams& p,
-                                          float* sig, float* cr, float* cg, float* cb) {
+template <int SPB, bool RNG>
+__device__ __forceinline__ void accum_hit(const Isect& I, float sigma, float4 col, float tb, unsigned int seed,
+                                          const DevParams& p, float* sig, float* cr, float* cg, float* cb) {
+    unsigned int st = seed;
 #pragma unroll
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
-        const float t = sample_t(tb, i, p.dt);
+        const float t = sample_t(tb, i, p.dt, RNG ? p.rnd_sample : 0, st);
         const float tau = t - I.tc;
         const float pw = fmaf(I.Cq, tau * tau, I.A);  // explicit: same rounding in every pass
         const float sw = sigma * __expf(-pw);
@@ -145,9 +146,10 @@ __device__ __forceinline__ void accum_hit(const Isect& I, float sigma, float4 co
 }
 
 // colorBlending (forward/gaussians_aabb.cu:500-519) + depth (cuda_gui/gaussians_aabb.cu:415).
-template <int SPB>
+template <int SPB, bool RNG>
 __device__ __forceinline__ void composite(const float* sig, const float* cr, const float* cg, const float* cb, float tb,
-                                          const DevParams& p, RayOut& o) {
+                                          unsigned int seed, const DevParams& p, RayOut& o) {
+    unsigned int st = seed;
 #pragma unroll
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
@@ -163,7 +165,7 @@ __device__ __forceinline__ void composite(const float* sig, const float* cr, con
         o.C[0] += ta * r;
         o.C[1] += ta * g;
         o.C[2] += ta * b;
-        o.depth += ta * sample_t(tb, i, p.dt);
+        o.depth += ta * sample_t(tb, i, p.dt, RNG ? p.rnd_sample : 0, st);
         o.T *= 1.0f - alpha;
     }
 }
@@ -203,17 +205,19 @@ __device__ __forceinline__ void slab_coeffs(float* sig, float* cr, float* cg, fl
 
 // Gradient of one (slab, hit): four sufficient statistics over the slab's samples, expanded
 // into v = (d mu[3], d sigma, d s[3], d q[4] (w,x,y,z), d rgb[3]).
-template <int SPB>
+template <int SPB, bool RNG>
 __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, float4 g1, float4 g2, float4 g3,
-                                          float4 col, float tb, const DevParams& p, const float* Aq, const float* Bq,
-                                          const float* Eq, const float* dL, const DevScene& sc, float* v) {
+                                          float4 col, float tb, unsigned int seed, const DevParams& p, const float* Aq,
+                                          const float* Bq, const float* Eq, const float* dL, const DevScene& sc,
+                                          float* v) {
     const float sigma = g0.w;
     const float F = dL[0] * col.x + dL[1] * col.y + dL[2] * col.z;
     float Ss = 0.0f, S1 = 0.0f, S2 = 0.0f, Sc = 0.0f;
+    unsigned int st = seed;
 #pragma unroll
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
-        const float t = sample_t(tb, i, p.dt);
+        const float t = sample_t(tb, i, p.dt, RNG ? p.rnd_sample : 0, st);
         const float tau = t - I.tc;
         const float pw = fmaf(I.Cq, tau * tau, I.A);  // explicit: same rounding in every pass
         const float w = __expf(-pw);
@@ -541,9 +545,9 @@ __device__ __forceinline__ void flush_counters(const Counters& c, DevStats st) {
 
 // ------------------------------------------------------------------ per-ray bodies -----------
 
-template <int SPB, int CAP>
-__device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& ray, const float4* __restrict__ rgbv,
-                            IList<CAP>& L, int lane, Counters& ctr, RayOut& o) {
+template <int SPB, int CAP, bool RNG>
+__device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& ray, unsigned int seed,
+                            const float4* __restrict__ rgbv, IList<CAP>& L, int lane, Counters& ctr, RayOut& o) {
     o.C[0] = o.C[1] = o.C[2] = 0.0f;
     o.depth = 0.0f;
     o.T = 1.0f;
@@ -562,7 +566,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
         for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
         int nh = 0;
         auto eval_hit = [&](int slot, const Isect& I, float sigma) {
-            accum_hit<SPB>(I, sigma, rgbv[slot], tb, p, sig, cr, cg, cb);
+            accum_hit<SPB, RNG>(I, sigma, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
             if (tb == first_tb || I.r1 > tmin) ctr.P++;
             ++nh;
         };
@@ -588,7 +592,10 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
         }
         ctr.K += nh;
         if (p.max_hits > 0 && nh > p.max_hits) ctr.ovf++;
-        if (nh > 0) composite<SPB>(sig, cr, cg, cb, tb, p, o);
+        if (nh > 0) {
+            composite<SPB, RNG>(sig, cr, cg, cb, tb, seed, p, o);
+            if (RNG && p.rnd_sample == 1) seed = skip_draws(seed, p.spb);  // the slab's draws are spent
+        }
         if (!advance_slab(s, p)) break;
     }
 }
@@ -597,9 +604,10 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
 // lanes together: a packet traversal over the lanes' slab intervals; at each leaf the lanes whose
 // slab holds the Gaussian compute its gradient contribution and the wave sums them into one
 // staged record per view (as in the list scatter), instead of 14 atomics per lane and hit.
-template <int SPB>
+template <int SPB, bool RNG>
 __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool dsc, const Ray& ray,
-                                    const float4* __restrict__ rgbv, float cur_tb, float cur_tmin, float cur_tmax,
+                                    const float4* __restrict__ rgbv, float cur_tb, unsigned int cur_seed,
+                                    float cur_tmin, float cur_tmax,
                                     const float* Aq, const float* Bq, const float* Eq, const float* dL, GradStage& G,
                                     int& nst, int lane, const DevGrad& gr, int view) {
     const float ix = 1.0f / ray.dx, iy = 1.0f / ray.dy, iz = 1.0f / ray.dz;
@@ -633,7 +641,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             Isect I;
             if (intersect(g0, g1, g2, g3, ray, I) && overlaps(cur_tmin, cur_tmax, I.r1, I.r2)) {
                 refine_sampling(g0, g1, g2, g3, ray, I);
-                grad_vals<SPB>(I, leaf, g0, g1, g2, g3, rgbv[leaf], cur_tb, p, Aq, Bq, Eq, dL, sc, v);
+                grad_vals<SPB, RNG>(I, leaf, g0, g1, g2, g3, rgbv[leaf], cur_tb, cur_seed, p, Aq, Bq, Eq, dL, sc, v);
                 c = true;
             }
         }
@@ -656,8 +664,8 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
 // (recomputed buffers, per-sample coefficients, T and Crem advanced); then the wave walks the
 // union of the prepared slabs' hits in increasing (view, slot) order and scatters one summed
 // contribution per Gaussian.  Slabs served by per-slab traversal scatter per lane.
-template <int SPB, int CAP>
-__device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray,
+template <int SPB, int CAP, bool RNG>
+__device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray, unsigned int seed,
                               const float4* __restrict__ rgbv, const float* Cfwd, const float* dL, IList<CAP>& L,
                               GradStage& G, int lane, const DevGrad& gr, int view, Counters& ctr) {
     int nst = 0;  // records in the staging area
@@ -677,6 +685,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         // prepared slab: list mode (has: cur_cnt list entries) or per-slab traversal (dsc)
         bool has = false, dsc = false;
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
+        unsigned int cur_seed = 0u;
         int cur_cnt = 0;
         if (alive && !pend) {
             if (!(T > p.t_eps)) {
@@ -706,7 +715,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         if (!intersect(g0, g1, g2, g3, ray, I)) return true;
                         if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
                         refine_sampling(g0, g1, g2, g3, ray, I);
-                        accum_hit<SPB>(I, g0.w, rgbv[slot], tb, p, sig, cr, cg, cb);
+                        accum_hit<SPB, RNG>(I, g0.w, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
                         ++nh;
                         return true;
                     });
@@ -716,8 +725,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         const int e = __builtin_ctz(m);
                         m &= m - 1u;
                         const int slot = L.g[e][lane];
-                        accum_hit<SPB>(entry_isect(L, e, lane), sc.geo[4 * slot].w, rgbv[slot], tb, p, sig, cr, cg,
-                                       cb);
+                        accum_hit<SPB, RNG>(entry_isect(L, e, lane), sc.geo[4 * slot].w, rgbv[slot], tb, seed, p, sig,
+                                       cr, cg, cb);
                         ++nh;
                     }
                 }
@@ -730,8 +739,10 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         Eq[i] = cg[i];
                     }
                     cur_tb = tb;
+                    cur_seed = seed;
                     cur_tmin = tmin;
                     cur_tmax = tmax;
+                    if (RNG && p.rnd_sample == 1) seed = skip_draws(seed, p.spb);  // the slab's draws are spent
                     if (s.direct) {
                         dsc = true;
                     } else {
@@ -744,8 +755,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         }
         // converged: slabs whose hit set overflowed the list scatter from a packet traversal
         if (__ballot(dsc) != 0ULL)
-            direct_scatter_wave<SPB>(sc, p, dsc, ray, rgbv, cur_tb, cur_tmin, cur_tmax, Aq, Bq, Eq, dL, G, nst, lane,
-                                     gr, view);
+            direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, Aq, Bq, Eq, dL, G,
+                                     nst, lane, gr, view);
         // wave-merged scatter over the prepared slabs' list hits
         int e = 0;
         while (true) {
@@ -772,7 +783,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 Isect I;
                 intersect(g0, g1, g2, g3, ray, I);
                 refine_sampling(g0, g1, g2, g3, ray, I);
-                grad_vals<SPB>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, p, Aq, Bq, Eq, dL, sc, v);
+                grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, Aq, Bq, Eq, dL, sc, v);
                 ++e;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
@@ -808,7 +819,7 @@ __device__ __forceinline__ int64_t block_item(int64_t b, int64_t nblocks) {
     return xcd * per + k;
 }
 
-template <int SPB, int CAP>
+template <int SPB, int CAP, bool RNG>
 __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
                                                    float* __restrict__ out_rgb, float* __restrict__ out_depth,
                                                    float* __restrict__ out_opac, DevStats st) {
@@ -820,9 +831,10 @@ __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, Ray
     RayRef q;
     if (item < num_items && resolve(job, item, q)) {
         const rg_camera cam = job.cams[q.view];
-        const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
+        unsigned int seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
+        const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
         RayOut o;
-        forward_ray<SPB, CAP>(sc, p, ray, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o);
+        forward_ray<SPB, CAP, RNG>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o);
         out_rgb[3 * q.out + 0] = o.C[0] + o.T * p.bg;
         out_rgb[3 * q.out + 1] = o.C[1] + o.T * p.bg;
         out_rgb[3 * q.out + 2] = o.C[2] + o.T * p.bg;
@@ -836,7 +848,7 @@ __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, Ray
 // per-pixel loss of the rendered colours against target (loss_kind 0: L2 sum (C-t)^2, 1: L1
 // sum |C-t|), whose value is summed into the loss counter.  count_stats = 0 leaves the
 // traversal counters to the forward of the same step.
-template <int SPB, int CAP>
+template <int SPB, int CAP, bool RNG>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) k_backward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
                                                     const float* __restrict__ ray_colors, const float* __restrict__ dLdC,
                                                     const float* __restrict__ target, int loss_kind, int count_stats,
@@ -850,6 +862,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
     RayRef q;
     bool active = false;
     Ray ray = {};
+    unsigned int seed = 0u;
     float dL[3] = {0.0f, 0.0f, 0.0f}, Cf[3] = {0.0f, 0.0f, 0.0f};
     float loss = 0.0f;
     int view = 0;
@@ -878,10 +891,12 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
             active = true;
             view = q.view;
             const rg_camera cam = job.cams[q.view];
-            ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
+            seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
+            ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
         }
     }
-    backward_wave<SPB, CAP>(sc, p, active, ray, sc.rgb + (size_t)view * sc.n, Cf, dL, L, G, lane, gr, view, ctr);
+    backward_wave<SPB, CAP, RNG>(sc, p, active, ray, seed, sc.rgb + (size_t)view * sc.n, Cf, dL, L, G, lane, gr, view,
+                            ctr);
     if (count_stats) flush_counters(ctr, st);
     if (target) {
         const float ls = wave_sum64(loss);
@@ -897,7 +912,8 @@ __global__ void __launch_bounds__(kWave) k_debug_hits(DevScene sc, DevParams p, 
     if (r >= job.num_rays) return;
     const RayRef q = ray_ref(job, r);
     const rg_camera cam = job.cams[q.view];
-    const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far);
+    unsigned int seed = tea4(p.iteration, (unsigned int)q.pixel);
+    const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, p.jitter, &seed);
     ro[3 * r] = ray.ox;
     ro[3 * r + 1] = ray.oy;
     ro[3 * r + 2] = ray.oz;
@@ -943,16 +959,22 @@ __global__ void k_reset_counters(unsigned long long* c) {
 }  // namespace
 
 // In the implicit single-view mode RayJob.num_rays carries the padded 8x8-tile item count.
-#define RG_DISPATCH(SPBV, CAPV, KERNEL, ...)                                                                         \
+#define RG_DISPATCH_CAP(SPBC, RNGC, CAPV, KERNEL, ...)                                                            \
+    do {                                                                                                             \
+        if (CAPV <= 8) hipLaunchKernelGGL((KERNEL<SPBC, 8, RNGC>), __VA_ARGS__);                                     \
+        else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<SPBC, 16, RNGC>), __VA_ARGS__);                              \
+        else hipLaunchKernelGGL((KERNEL<SPBC, 32, RNGC>), __VA_ARGS__);                                              \
+    } while (0)
+// Kernels are specialised on SPB (8 or 16 samples per slab), the list capacity and on whether
+// the random generator is used at all (jitter or rnd_sample), which costs registers.
+#define RG_DISPATCH(SPBV, CAPV, RNGV, KERNEL, ...)                                                                   \
     do {                                                                                                             \
         if (SPBV == 8) {                                                                                             \
-            if (CAPV <= 8) hipLaunchKernelGGL((KERNEL<8, 8>), __VA_ARGS__);                                          \
-            else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<8, 16>), __VA_ARGS__);                                   \
-            else hipLaunchKernelGGL((KERNEL<8, 32>), __VA_ARGS__);                                                   \
+            if (RNGV) RG_DISPATCH_CAP(8, true, CAPV, KERNEL, __VA_ARGS__);                                           \
+            else RG_DISPATCH_CAP(8, false, CAPV, KERNEL, __VA_ARGS__);                                               \
         } else {                                                                                                     \
-            if (CAPV <= 8) hipLaunchKernelGGL((KERNEL<16, 8>), __VA_ARGS__);                                         \
-            else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<16, 16>), __VA_ARGS__);                                  \
-            else hipLaunchKernelGGL((KERNEL<16, 32>), __VA_ARGS__);                                                  \
+            if (RNGV) RG_DISPATCH_CAP(16, true, CAPV, KERNEL, __VA_ARGS__);                                          \
+            else RG_DISPATCH_CAP(16, false, CAPV, KERNEL, __VA_ARGS__);                                              \
         }                                                                                                            \
     } while (0)
 
@@ -967,8 +989,9 @@ hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& 
     if (items <= 0) return hipSuccess;
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
-    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job, items,
-                out_rgb, out_depth, out_opacity, st);
+    const bool rng = p.jitter || p.rnd_sample;
+    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, rng, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
+                items, out_rgb, out_depth, out_opacity, st);
     return hipGetLastError();
 }
 
@@ -979,7 +1002,8 @@ hipError_t launch_backward(const DevScene& sc, const DevParams& p, const RayJob&
     if (items <= 0) return hipSuccess;
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
-    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, k_backward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
+    const bool rng = p.jitter || p.rnd_sample;
+    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, rng, k_backward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
                 items, ray_colors, dLdC, target, loss_kind, count_stats, gr, st);
     return hipGetLastError();
 }

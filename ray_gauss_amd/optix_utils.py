@@ -29,7 +29,8 @@ class DeviceContext:
 
     def caster_for(self, params, max_prim_slice=0, exact_sh_grad=False) -> RayCaster:
         key = (params.dt, params.samples_per_slab, params.t_eps, params.sigma_thr, params.t_near, params.t_far,
-               bool(params.white_background), int(max_prim_slice), bool(exact_sh_grad))
+               bool(params.white_background), int(max_prim_slice), bool(exact_sh_grad), int(params.jitter),
+               int(params.rnd_sample), int(params.iteration))
         if self._caster is None:
             self._caster = RayCaster(params, self.tune, exact_sh_grad, max_prim_slice)
         elif key != self._key:

@@ -15,7 +15,9 @@ Differences a caller can see:
 * ``max_prim_slice`` no longer truncates hit sets: every hit is kept; slabs with more hits than
   ``max_prim_slice`` are counted and reported like the reference's overflow counter
   (``optix_utils.py:205-207``).
-* ``jitter`` / ``rnd_sample`` other than 0 raise ``NotImplementedError``.
+* ``jitter`` and ``rnd_sample`` follow the reference exactly, including its random generator
+  (``tea<4>(iteration, pixel)`` seeds, LCG draws; ``forward/gaussians_aabb.h:63-94``); the
+  backward of a call reuses that call's iteration.
 * ``return_aux=True`` makes ``apply`` return ``(ray_colors, depth, opacity)`` with
   depth = sum T alpha t and opacity = 1 - T_final; the default return is unchanged.
 """
@@ -58,7 +60,8 @@ def _render_params(settings) -> RenderParams:
     base = settings.params if settings.params is not None else RenderParams()
     return RenderParams(dt=base.dt, samples_per_slab=base.samples_per_slab, t_eps=base.t_eps,
                         sigma_thr=base.sigma_thr, t_near=base.t_near, t_far=base.t_far,
-                        white_background=bool(settings.white_background))
+                        white_background=bool(settings.white_background), jitter=int(settings.jitter or 0),
+                        rnd_sample=int(settings.rnd_sample or 0), iteration=int(settings.iteration or 0))
 
 
 def _caster(settings):
@@ -87,8 +90,8 @@ class RenderOptixFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, positions, scales, normalized_quaternions, densities, color_features, sph_gauss_features,
                 bandwidth_sharpness, lobe_axis, settings):
-        if settings.jitter or settings.rnd_sample:
-            raise NotImplementedError("jitter and rnd_sample are not supported by the gfx950 ray caster")
+        if int(settings.jitter or 0) not in (0, 1) or int(settings.rnd_sample or 0) not in (0, 1, 2):
+            raise ValueError("jitter must be 0 or 1 and rnd_sample 0, 1 or 2")
         caster = _caster(settings)
         L = int(settings.num_sph_gauss)
         sg = sph_gauss_features if L else None
@@ -109,6 +112,7 @@ class RenderOptixFunction(torch.autograd.Function):
         ctx.caster = caster
         ctx.cam = cam
         ctx.generation = caster.generation
+        ctx.params = caster.params
         ctx.num_sg = L
         ctx.save_for_backward(positions, scales, normalized_quaternions, densities, color_features,
                               sph_gauss_features, bandwidth_sharpness, lobe_axis, rgb.clone())
@@ -122,6 +126,10 @@ class RenderOptixFunction(torch.autograd.Function):
         (positions, scales, quats, densities, color_features, sg, sharp, axis, ray_colors) = ctx.saved_tensors
         caster = ctx.caster
         L = ctx.num_sg
+        if caster.params != ctx.params:
+            # another call (e.g. a later iteration) changed the handle's settings: restore this one's
+            caster.params = ctx.params
+            caster.set_tune(caster.tune)
         if caster.generation != ctx.generation:
             # another forward used this handle in between: restore this call's state
             caster.prepare(positions, scales, quats, densities, color_features, sg if L else None,

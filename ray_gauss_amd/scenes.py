@@ -67,7 +67,9 @@ class Scene:
 
 @dataclass
 class RenderParams:
-    """Runtime replacements of the reference's compile-time constants (gaussians_aabb.h:2-5)."""
+    """Runtime replacements of the reference's compile-time constants (gaussians_aabb.h:2-5), plus
+    the per-call sampling settings of RenderOptixSettings (render_optix.py:67-87): jitter,
+    rnd_sample and the iteration that seeds the per-ray random generator."""
 
     dt: float = 0.0025
     samples_per_slab: int = 8
@@ -76,6 +78,9 @@ class RenderParams:
     t_near: float = 0.0
     t_far: float = float("inf")
     white_background: bool = False
+    jitter: int = 0
+    rnd_sample: int = 0
+    iteration: int = 0
 
 
 @dataclass

@@ -182,13 +182,25 @@ def test_dropin_autograd_boundary(gpu_device):
     assert max_rel(rgb_dc.grad.cpu().numpy(), g["sh"][:, :, 0].cpu().numpy()) <= 1e-5
 
 
-def test_dropin_rejects_jitter(gpu_device):
+def test_dropin_jitter_and_random_samples(gpu_device):
+    """The drop-in accepts the reference's jitter / rnd_sample settings; the image depends on the
+    iteration that seeds the generator, and its gradient equals the caster's for the same
+    settings."""
     from ray_gauss_amd import render as r_ox
+    from ray_gauss_amd.caster import RayCaster
 
     wl = scenes.workload("small4k")
-    settings = r_ox.RenderOptixSettings(None, False, None, None, None, None, None, wl.cameras[0], 1024, 0, 0, 1, 0,
-                                        (1, 1), False, None)
     t = to_dev(wl.scene, gpu_device)
-    with pytest.raises(NotImplementedError):
-        r_ox.RenderOptixFunction.apply(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"], t["sg"],
-                                       t["sharpness"], t["lobe_axis"], settings)
+    imgs = []
+    for it in (5, 6):
+        settings = r_ox.RenderOptixSettings(None, False, None, None, None, None, None, wl.cameras[0], 1024, 0, it,
+                                            1, 1, (1, 1), False, None, params=wl.params)
+        imgs.append(r_ox.RenderOptixFunction.apply(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"],
+                                                   t["sg"], t["sharpness"], t["lobe_axis"], settings))
+    assert not torch.equal(imgs[0], imgs[1])
+    prm = scenes.RenderParams(**{**wl.params.__dict__, "jitter": 1, "rnd_sample": 1, "iteration": 6})
+    rc = RayCaster(prm)
+    rc.prepare(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"], rebuild=True)
+    rc.set_views([wl.cameras[0].launch_record()])
+    rgb, _, _ = rc.forward(0)
+    assert torch.equal(rgb, imgs[1])

@@ -385,3 +385,78 @@ def test_backward_matches_central_fd():
             assert abs(num - ana) <= 1e-3 * scale, (name, idx, num, ana)
             checked += 1
     assert checked >= 30
+
+
+# ---- random generator of jitter / rnd_sample (forward/gaussians_aabb.h:63-94) ----------------
+
+def test_tea_is_the_published_cipher():
+    """With 32 cycles and an all-zero key and block, TEA's published test vector is 0x41ea3a0a
+    for the first word (Wheeler & Needham's cipher; the reference's tea<N> uses the same rounds
+    with a fixed key and N cycles)."""
+    assert O.tea(0, 0, 32, [0, 0, 0, 0]) == 0x41EA3A0A
+
+
+def test_tea4_uses_the_reference_key():
+    key = [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E]
+    for v0, v1 in [(0, 0), (7, 123456), (2 ** 32 - 1, 99)]:
+        assert O.tea4(v0, v1) == O.tea(v0, v1, 4, key)
+    assert len({O.tea4(5, px) for px in range(1000)}) == 1000  # distinct seeds per pixel
+
+
+def test_lcg_closed_form():
+    """x_{k+1} = 1664525 x_k + 1013904223 mod 2^32 (Numerical Recipes' generator); the output is
+    the low 24 bits, and rnd = output / 2^24 is exact in float32."""
+    ints, floats = O.lcg(12345, 50)
+    x = 12345
+    for k in range(50):
+        x = (1664525 * x + 1013904223) % 2 ** 32
+        assert ints[k] == x & 0xFFFFFF
+        assert floats[k] == np.float32(ints[k]) / np.float32(2 ** 24)
+    assert floats.min() >= 0.0 and floats.max() < 1.0
+
+
+def test_jitter_moves_rays_within_the_pixel():
+    wl = scenes.workload("small4k")
+    cam = wl.cameras[0].launch_record()
+    o0, d0 = O.rays(cam)
+    o1, d1 = O.rays_rng(cam, None, jitter=0, iteration=3)
+    assert np.array_equal(d0, d1)  # no jitter: the generator is seeded but unused
+    o2, d2 = O.rays_rng(cam, None, jitter=1, iteration=3)
+    o3, d3 = O.rays_rng(cam, None, jitter=1, iteration=4)
+    assert not np.array_equal(d2, d0) and not np.array_equal(d2, d3)
+    # the jittered sample point stays inside the pixel: [x + 1/3, x + 2/3)
+    W, H = cam["width"], cam["height"]
+    u = np.asarray(cam["u"]); v = np.asarray(cam["v"]); w = np.asarray(cam["w"])
+    dd = d2.astype(np.float64)
+    a = dd @ u / (dd @ w) / cam["tanx"]
+    b = dd @ v / (dd @ w) / cam["tany"]
+    fx = (a + 1) / 2 * W - np.tile(np.arange(W), H)
+    fy = (b + 1) / 2 * H - np.repeat(np.arange(H), W)
+    assert fx.min() > 0.333 - 1e-3 and fx.max() < 0.667 + 1e-3
+    assert fy.min() > 0.333 - 1e-3 and fy.max() < 0.667 + 1e-3
+
+
+def test_sample_offsets_keep_the_single_gaussian_closed_form():
+    """With random (rnd_sample = 1) or midpoint (2) sample offsets the samples move, but one
+    Gaussian's image still telescopes to C = rgb (1 - T); random offsets depend on the iteration
+    that seeds the generator, midpoint offsets do not."""
+    wl = scenes.workload("small4k")
+    cam = wl.cameras[0].launch_record()
+    sc = wl.scene
+    one = scenes.Scene(positions=sc.positions[:1] * 0, scales=np.full((1, 3), 0.2, np.float32),
+                       quats=np.array([[1, 0, 0, 0]], np.float32), densities=np.array([3.0], np.float32),
+                       sh=sc.sh[:1], sg=sc.sg[:1], sharpness=sc.sharpness[:1], lobe_axis=sc.lobe_axis[:1])
+    px = np.array([32 * 64 + 32, 30 * 64 + 33], np.int32)
+    for mode in (1, 2):
+        prm = scenes.RenderParams(**{**wl.params.__dict__, "rnd_sample": mode, "iteration": 11})
+        bmin, bmax = O.scene_box(one, prm.sigma_thr)
+        rgb = O.sh_forward(one, cam["eye"], 1)
+        out = O.render(prm, one, rgb, bmin, bmax, cam, px, 1)
+        T = 1.0 - out["opacity"]
+        assert T.max() < 0.99
+        np.testing.assert_allclose(out["rgb"], rgb[0][None, :] * (1.0 - T[:, None]), rtol=1e-12, atol=1e-12)
+        prm2 = scenes.RenderParams(**{**prm.__dict__, "iteration": 12})
+        out2 = O.render(prm2, one, rgb, bmin, bmax, cam, px, 1)
+        base = O.render(scenes.RenderParams(**{**prm.__dict__, "rnd_sample": 0}), one, rgb, bmin, bmax, cam, px, 1)
+        assert not np.allclose(out["rgb"], base["rgb"], rtol=0, atol=1e-9)
+        assert np.array_equal(out["rgb"], out2["rgb"]) == (mode == 2)
