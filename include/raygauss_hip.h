@@ -52,6 +52,8 @@ typedef struct rg_config {
     int32_t jitter;            /* RenderOptixSettings.jitter: sub-pixel jitter (gaussians_aabb.cu:409) */
     int32_t rnd_sample;        /* RenderOptixSettings.rnd_sample: 0 grid, 1 random, 2 mid-sample     */
     uint32_t iteration;        /* RenderOptixSettings.iteration: seeds tea<4>(iteration, pixel)       */
+    int32_t fwd_hit_depth;     /* list capacity of the forward kernel: 6, 8, 16, 32 (0: hit_depth);
+                                  a smaller list lets more wavefronts share a CU's LDS            */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

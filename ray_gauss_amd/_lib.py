@@ -20,7 +20,8 @@ class Config(C.Structure):
                 ("t_near", C.c_float), ("t_far", C.c_float), ("white_background", C.c_int32),
                 ("exact_sh_grad", C.c_int32), ("hit_depth", C.c_int32), ("seg_slabs", C.c_int32),
                 ("max_hits_per_slab", C.c_int32), ("packet_traversal", C.c_int32),
-                ("jitter", C.c_int32), ("rnd_sample", C.c_int32), ("iteration", C.c_uint32)]
+                ("jitter", C.c_int32), ("rnd_sample", C.c_int32), ("iteration", C.c_uint32),
+                ("fwd_hit_depth", C.c_int32)]
 
 
 class CameraS(C.Structure):

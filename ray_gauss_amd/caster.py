@@ -20,12 +20,13 @@ from .cameras import launch_record_from
 
 @dataclass
 class TuneConfig:
-    """Launch tuning: interval-list depth per ray, initial slabs per gather, and whether a
-    wavefront's lanes walk the LBVH as one packet."""
+    """Launch tuning: interval-list depth per ray, initial slabs per gather, whether a wavefront's
+    lanes walk the LBVH as one packet, and the forward kernel's own (smaller) list depth."""
 
     hit_depth: int = 8
     seg_slabs: int = 8
     packet: bool = True
+    fwd_hit_depth: int = 6
 
 
 def _ptr(t):
@@ -103,6 +104,7 @@ class RayCaster:
         c.seg_slabs = int(self.tune.seg_slabs)
         c.max_hits_per_slab = int(self.max_hits_per_slab)
         c.packet_traversal = int(bool(self.tune.packet))
+        c.fwd_hit_depth = int(self.tune.fwd_hit_depth)
         c.jitter = int(getattr(p, "jitter", 0))
         c.rnd_sample = int(getattr(p, "rnd_sample", 0))
         c.iteration = int(getattr(p, "iteration", 0) or 0) & 0xFFFFFFFF

@@ -90,6 +90,9 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->hit_depth != 8 && c->hit_depth != 16 && c->hit_depth != 32)
         return fail(h, RG_ERR_ARG, "hit_depth must be 8, 16 or 32");
     if (c->seg_slabs < 1 || c->seg_slabs > 4096) return fail(h, RG_ERR_ARG, "seg_slabs in 1..4096");
+    if (c->fwd_hit_depth != 0 && c->fwd_hit_depth != 6 && c->fwd_hit_depth != 8 && c->fwd_hit_depth != 16 &&
+        c->fwd_hit_depth != 32)
+        return fail(h, RG_ERR_ARG, "fwd_hit_depth must be 0, 6, 8, 16 or 32");
     if (c->jitter != 0 && c->jitter != 1) return fail(h, RG_ERR_ARG, "jitter must be 0 or 1");
     if (c->rnd_sample < 0 || c->rnd_sample > 2) return fail(h, RG_ERR_ARG, "rnd_sample must be 0, 1 or 2");
     return RG_OK;
@@ -199,6 +202,8 @@ rg::RayJob single_job(rg_handle* h, int view, int64_t num_rays, const int32_t* p
     }
     return j;
 }
+
+int fwd_depth(const rg_handle* h) { return h->cfg.fwd_hit_depth ? h->cfg.fwd_hit_depth : h->cfg.hit_depth; }
 
 rg::DevGrad dev_grad(rg_handle* h) {
     rg::DevGrad g;
@@ -328,7 +333,7 @@ int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixe
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
-    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, out_rgb, out_depth, out_opacity,
+    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), job, fwd_depth(h), out_rgb, out_depth, out_opacity,
                                  dev_stats(h), s));
     return RG_OK;
 }
@@ -374,7 +379,7 @@ int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float*
     if ((rc = batch_job(h, b, j))) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
-    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, out_rgb, out_depth, out_opacity,
+    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), j, fwd_depth(h), out_rgb, out_depth, out_opacity,
                                  dev_stats(h), s));
     return RG_OK;
 }
@@ -412,7 +417,7 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
     }
     const rg::DevScene sc = dev_scene(h);
     const rg::DevParams p = dev_params(h);
-    HIPCHK(h, rg::launch_forward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, nullptr, dev_stats(h), s));
+    HIPCHK(h, rg::launch_forward(sc, p, j, fwd_depth(h), rgb, nullptr, nullptr, dev_stats(h), s));
     HIPCHK(h, rg::launch_backward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, target, loss_kind, 0, dev_grad(h),
                                   dev_stats(h), s));
     if (out_loss)

@@ -965,18 +965,25 @@ __global__ void k_reset_counters(unsigned long long* c) {
         else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<SPBC, 16, RNGC>), __VA_ARGS__);                              \
         else hipLaunchKernelGGL((KERNEL<SPBC, 32, RNGC>), __VA_ARGS__);                                              \
     } while (0)
+#define RG_DISPATCH_CAP6(SPBC, RNGC, CAPV, KERNEL, ...)                                                           \
+    do {                                                                                                             \
+        if (CAPV <= 6) hipLaunchKernelGGL((KERNEL<SPBC, 6, RNGC>), __VA_ARGS__);                                     \
+        else RG_DISPATCH_CAP(SPBC, RNGC, CAPV, KERNEL, __VA_ARGS__);                                                 \
+    } while (0)
 // Kernels are specialised on SPB (8 or 16 samples per slab), the list capacity and on whether
 // the random generator is used at all (jitter or rnd_sample), which costs registers.
-#define RG_DISPATCH(SPBV, CAPV, RNGV, KERNEL, ...)                                                                   \
+#define RG_DISPATCH_ANY(CAPMACRO, SPBV, CAPV, RNGV, KERNEL, ...)                                                     \
     do {                                                                                                             \
         if (SPBV == 8) {                                                                                             \
-            if (RNGV) RG_DISPATCH_CAP(8, true, CAPV, KERNEL, __VA_ARGS__);                                           \
-            else RG_DISPATCH_CAP(8, false, CAPV, KERNEL, __VA_ARGS__);                                               \
+            if (RNGV) CAPMACRO(8, true, CAPV, KERNEL, __VA_ARGS__);                                                  \
+            else CAPMACRO(8, false, CAPV, KERNEL, __VA_ARGS__);                                                      \
         } else {                                                                                                     \
-            if (RNGV) RG_DISPATCH_CAP(16, true, CAPV, KERNEL, __VA_ARGS__);                                          \
-            else RG_DISPATCH_CAP(16, false, CAPV, KERNEL, __VA_ARGS__);                                              \
+            if (RNGV) CAPMACRO(16, true, CAPV, KERNEL, __VA_ARGS__);                                                 \
+            else CAPMACRO(16, false, CAPV, KERNEL, __VA_ARGS__);                                                     \
         }                                                                                                            \
     } while (0)
+#define RG_DISPATCH(...) RG_DISPATCH_ANY(RG_DISPATCH_CAP, __VA_ARGS__)
+#define RG_DISPATCH_FWD(...) RG_DISPATCH_ANY(RG_DISPATCH_CAP6, __VA_ARGS__)
 
 hipError_t launch_reset_counters(DevStats st, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, st.c);
@@ -990,7 +997,7 @@ hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& 
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
     const bool rng = p.jitter || p.rnd_sample;
-    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, rng, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
+    RG_DISPATCH_FWD(p.spb <= 8 ? 8 : 16, hit_depth, rng, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
                 items, out_rgb, out_depth, out_opacity, st);
     return hipGetLastError();
 }

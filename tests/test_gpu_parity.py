@@ -107,13 +107,15 @@ def test_backward_parity_small(small):
 
 @pytest.mark.parametrize("depth,seg,packet", [(8, 1, False), (32, 64, True), (16, 8, False)])
 def test_tuning_invariance(gpu_device, depth, seg, packet):
+    # the reference case uses the 6-entry forward list, the others the backward's depth
     """Hit-buffer depth, segment length and packet traversal change the schedule, not the image:
     every schedule sums each slab's hits in the same (Morton slot) order, so the images are
     bit-identical."""
     wl = scenes.workload("small4k")
     cam = wl.cameras[0].launch_record()
     rc0, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device, TuneConfig(16, 8, packet=True))
-    rc1, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device, TuneConfig(depth, seg, packet=packet))
+    rc1, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device,
+                             TuneConfig(depth, seg, packet=packet, fwd_hit_depth=depth))
     a = rc0.forward(0)[0].cpu().numpy()
     b = rc1.forward(0)[0].cpu().numpy()
     assert np.array_equal(a, b), np.abs(a - b).max()
