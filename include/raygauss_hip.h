@@ -135,6 +135,11 @@ int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* strea
  * camera part of the launch record (optix_utils.py:166-181). */
 int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* stream);
 
+/* Replaces the colour table of one view (after rg_set_views) with caller-computed per-Gaussian
+ * colours rgb (n,3), original order: the test path renders colours computed outside the caster
+ * (compute_cupy_rgb -> launch_pipeline_test, scripts/test.py:107-116, optix_utils.py:302-373). */
+int rg_set_view_colors(rg_handle* h, int32_t view, const float* rgb, void* stream);
+
 /* Forward render of num_rays rays of one view (pixels == NULL: ray r is pixel r).  out_rgb
  * (num_rays,3) is required; out_depth / out_opacity (num_rays) are optional.  Replaces
  * launch_pipeline_forward (optix_utils.py:128-209) and the forward raygen
@@ -236,6 +241,10 @@ typedef struct rg_projection {
     int32_t normalize_lobe_axis;
 } rg_projection;
 int rg_project_params(const rg_projection* pr, void* stream);
+
+/* Viewer output (handle-free): n colours (n,3) -> RGBA8 (n,4) as the viewer's make_color
+ * (cuda_gui/helpers.h:35-57): clamp to [0,1], min(floor(256 x), 255), alpha 255. */
+int rg_colors_to_rgba8(const float* rgb, int64_t n, uint8_t* rgba, void* stream);
 
 /* ---- photometric loss (handle-free) -------------------------------------------------------- */
 

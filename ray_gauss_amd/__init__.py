@@ -6,6 +6,7 @@ Public surface:
   RayCaster, TuneConfig                     -- direct handle on the HIP library
   FusedAdam, step_all, project_params       -- optimiser step + post-step projections (optim.py)
   l1_ssim_loss                              -- the training loss and its gradient (loss.py)
+  viewer, formats                           -- RGBA8 + depth frames; RayGauss PLY / checkpoints
   scenes, cameras                           -- synthetic workloads and pinhole cameras
 """
 

@@ -84,6 +84,8 @@ EXPORTS = {
     "rg_debug_check_bvh": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
     "rg_adam_step": (C.c_int, [C.POINTER(AdamTensor), C.c_int32, C.c_void_p]),
     "rg_project_params": (C.c_int, [C.POINTER(Projection), C.c_void_p]),
+    "rg_set_view_colors": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "rg_colors_to_rgba8": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "rg_loss_workspace_bytes": (C.c_int64, [C.c_int32, C.c_int32]),
     "rg_loss_l1_ssim": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),

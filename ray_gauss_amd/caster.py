@@ -165,6 +165,14 @@ class RayCaster:
         _lib.check(self._L.rg_set_views(self._h, len(cams), arr, _stream()), self._h)
         self._cams = [c if isinstance(c, dict) else launch_record_from(c) for c in cams]
 
+    def set_view_colors(self, view: int, rgb):
+        """Replace view `view`'s per-Gaussian colours with rgb (n,3) computed by the caller."""
+        if rgb.numel() != 3 * self.n:
+            raise ValueError(f"set_view_colors: expected {self.n} x 3 colours, got shape {tuple(rgb.shape)}")
+        rgb = _f32c(rgb, (self.n, 3))
+        _lib.check(self._L.rg_set_view_colors(self._h, int(view), C.c_void_p(rgb.data_ptr()), _stream()), self._h)
+        self._keep_colors = rgb
+
     @property
     def views(self):
         return self._cams

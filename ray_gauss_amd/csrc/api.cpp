@@ -321,6 +321,15 @@ int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* s
     return RG_OK;
 }
 
+int rg_set_view_colors(rg_handle* h, int32_t view, const float* rgb, void* stream) {
+    if (!h || !rgb) return fail(h, RG_ERR_ARG, "null argument");
+    if (!h->built || h->num_views <= 0) return fail(h, RG_ERR_STATE, "rg_set_views has not been called");
+    if (view < 0 || view >= h->num_views) return fail(h, RG_ERR_ARG, "view out of range");
+    HIPCHK(h, rg::launch_set_colors((int)h->n, rgb, h->bvh.iperm, h->rgb + (size_t)view * h->n,
+                                    static_cast<hipStream_t>(stream)));
+    return RG_OK;
+}
+
 int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, float* out_rgb, float* out_depth,
                float* out_opacity, void* stream) {
     if (!h || !out_rgb) return fail(h, RG_ERR_ARG, "null argument");
@@ -556,6 +565,13 @@ int rg_project_params(const rg_projection* pr, void* stream) {
         return fail_free(RG_ERR_ARG, "rg_project_params: null lobe_axis with num_sg > 0");
     const hipError_t e = rg::launch_project(*pr, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_project_params: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int rg_colors_to_rgba8(const float* rgb, int64_t n, uint8_t* rgba, void* stream) {
+    if (n < 0 || (n > 0 && (!rgb || !rgba))) return fail_free(RG_ERR_ARG, "rg_colors_to_rgba8: bad argument");
+    const hipError_t e = rg::launch_rgba8(n, rgb, reinterpret_cast<uchar4*>(rgba), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_colors_to_rgba8: ") + hipGetErrorString(e));
     return RG_OK;
 }
 

@@ -316,6 +316,29 @@ __global__ void __launch_bounds__(kBlock) k_finish(int n, int deg, int L, const 
     reinterpret_cast<float4*>(out.quats)[g] = a2;
 }
 
+// Caller-supplied colour table of one view: rgb (n,3) in original order -> slot order.
+__global__ void __launch_bounds__(kBlock) k_set_colors(int n, const float* __restrict__ rgb,
+                                                       const int* __restrict__ iperm, float4* __restrict__ table) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    table[iperm[g]] = make_float4(rgb[3 * g], rgb[3 * g + 1], rgb[3 * g + 2], 0.0f);
+}
+
+// make_color of the viewer (cuda_gui/helpers.h:35-57, the sRGB step commented out there):
+// clamp to [0, 1] as max(0, min(x, 1)) (so NaN -> 1), quantise x -> min(floor(256 x), 255),
+// alpha 255.
+__global__ void __launch_bounds__(kBlock) k_rgba8(int64_t n, const float* __restrict__ rgb,
+                                                  uchar4* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    unsigned char q[3];
+    for (int c = 0; c < 3; ++c) {
+        const float x = fmaxf(0.0f, fminf(rgb[3 * i + c], 1.0f));
+        q[c] = (unsigned char)min((unsigned int)(x * 256.0f), 255u);
+    }
+    out[i] = make_uchar4(q[0], q[1], q[2], 255);
+}
+
 __global__ void k_zero(float4* p, int64_t count4) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count4) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -368,6 +391,18 @@ hipError_t launch_finish_grads(const rg_gaussians& g, const int* iperm, const rg
     hipLaunchKernelGGL(k_finish, dim3(blocks(g.n, kBlock)), dim3(kBlock), 0, s, (int)g.n, g.sh_degree, g.num_sg,
                        g.positions, g.sh, g.sg, g.sharpness, g.lobe_axis, iperm, cams_dev, num_views, gr, exact,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_colors(int n, const float* rgb, const int* iperm, float4* table, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_colors, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, rgb, iperm, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_rgba8(int64_t n, const float* rgb, uchar4* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rgba8, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, n, rgb, out);
     return hipGetLastError();
 }
 

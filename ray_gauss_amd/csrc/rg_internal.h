@@ -75,6 +75,8 @@ hipError_t launch_sh_forward(const rg_gaussians& g, const int* iperm, const rg_c
 hipError_t launch_finish_grads(const rg_gaussians& g, const int* iperm, const rg_camera* cams_dev, int num_views,
                                const DevGrad& gr, int exact, const rg_grads& out, hipStream_t s);
 hipError_t launch_zero(float* p, int64_t count, hipStream_t s);
+hipError_t launch_set_colors(int n, const float* rgb, const int* iperm, float4* table, hipStream_t s);
+hipError_t launch_rgba8(int64_t n, const float* rgb, uchar4* out, hipStream_t s);
 hipError_t launch_export_predicate(int n, const int* iperm, const float4* geo, float* mu, float* Mp, int* valid,
                                    hipStream_t s);
 

@@ -676,9 +676,6 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
     seg_init(s, ray, p);
     const int key_base = view * sc.n;
     const float window = p.dt * (float)p.spb;  // lanes within one slab of the laggard step together
-    float Aq[SPB], Bq[SPB], Eq[SPB];
-#pragma unroll
-    for (int i = 0; i < SPB; ++i) Aq[i] = Bq[i] = Eq[i] = 0.0f;
     bool pend = false;  // next_slab() has produced a slab not yet processed
     float tmin = 0.0f, tmax = 0.0f;
     while (__ballot(alive) != 0ULL) {
@@ -687,6 +684,9 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
         unsigned int cur_seed = 0u;
         int cur_cnt = 0;
+        // the prepared slab's sample buffers; after slab_coeffs sig, cr, cg hold the per-sample
+        // backward coefficients A_i, Bq_i, E_i read by the scatter below
+        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
         if (alive && !pend) {
             if (!(T > p.t_eps)) {
                 alive = false;
@@ -703,7 +703,6 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
             pend = false;
             {
                 const float tb = s.tb;
-                float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
 #pragma unroll
                 for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
                 int nh = 0;
@@ -732,12 +731,6 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 }
                 if (nh > 0) {
                     slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
-#pragma unroll
-                    for (int i = 0; i < SPB; ++i) {
-                        Aq[i] = sig[i];
-                        Bq[i] = cr[i];
-                        Eq[i] = cg[i];
-                    }
                     cur_tb = tb;
                     cur_seed = seed;
                     cur_tmin = tmin;
@@ -755,7 +748,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         }
         // converged: slabs whose hit set overflowed the list scatter from a packet traversal
         if (__ballot(dsc) != 0ULL)
-            direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, Aq, Bq, Eq, dL, G,
+            direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, cg, dL, G,
                                      nst, lane, gr, view);
         // wave-merged scatter over the prepared slabs' list hits
         int e = 0;
@@ -783,7 +776,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 Isect I;
                 intersect(g0, g1, g2, g3, ray, I);
                 refine_sampling(g0, g1, g2, g3, ray, I);
-                grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, Aq, Bq, Eq, dL, sc, v);
+                grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc, v);
                 ++e;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;

@@ -54,7 +54,7 @@ def create_context(log=None, tune: TuneConfig | None = None) -> DeviceContext:
 def create_module(ctx, pipeline_opts, stage):
     if stage not in _STAGES:
         raise ValueError("stage must be 'forward', 'backward', 'test' or 'gui'")
-    return _Token("module", stage=stage)
+    return _Token("module", stage=stage, ctx=ctx)
 
 
 def create_program_groups(ctx, module):
@@ -75,7 +75,36 @@ def update_acceleration_structure(gas, bboxes):
 
 
 def create_sbt(program_grps, positions, scales, quaternions):
-    return _Token("sbt")
+    return _Token("sbt", positions=positions, scales=scales, quaternions=quaternions)
+
+
+def launch_pipeline_test(pipeline, sbt, gas, bbox_min, bbox_max, camera, densities, color_features, positions,
+                         scales, quaternions, max_prim_slice, rnd_sample, supersampling, white_background):
+    """Forward render of one camera with caller-computed per-Gaussian colours, no gradients
+    (optix_utils.py:302-373, called from scripts/test.py:112-116): returns ray_colors
+    (H*W*ssx*ssy, 3) as a torch tensor.  Inputs are torch CUDA tensors (the reference passes
+    CuPy views of the same data); color_features is the (n,3) colour table of compute_cupy_rgb.
+    The scene box is recomputed from the Gaussians (bbox_min / bbox_max are accepted for the
+    signature; the reference derives them from the same AABBs).  As in the reference's test
+    kernel (cuda_test/gaussians_aabb.cu:312-333, 455-490), rays go through pixel centres and
+    samples sit at slab start + i*dt whatever rnd_sample says: the argument is accepted and
+    ignored there too."""
+    from .render import _launch_camera
+    from .scenes import RenderParams
+
+    ctx = pipeline.groups[0].module.ctx
+    params = RenderParams(white_background=bool(white_background))
+    caster = ctx.caster_for(params, int(max_prim_slice or 0))
+    n = int(positions.reshape(-1, 3).shape[0])
+    f = lambda t, *s: t.reshape(*s).float().contiguous()  # noqa: E731
+    sh0 = torch.zeros((n, 3, 1), dtype=torch.float32, device=positions.device)
+    with torch.no_grad():
+        caster.prepare(f(positions, n, 3), f(scales, n, 3), f(quaternions, n, 4), f(densities, n), sh0, rebuild=True)
+        settings = _Token("settings", viewpoint_cam=camera, supersampling=supersampling)
+        caster.set_views([_launch_camera(settings)])
+        caster.set_view_colors(0, f(color_features, n, 3))
+        rgb, _, _ = caster.forward(0, None)
+    return rgb
 
 
 def quaternion_to_rotation(quaternion):
