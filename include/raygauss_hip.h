@@ -262,6 +262,94 @@ int rg_loss_l1_ssim(const float* ray_colors, const float* target, int32_t width,
                     int32_t ss_y, float lambda_ssim, float* out_loss, float* out_parts, float* dL_dC,
                     void* workspace, int64_t workspace_bytes, void* stream);
 
+/* ---- densification and initialisation utilities (handle-free) ------------------------------ */
+
+/* Device workspace bytes rg_knn_mean_dist needs for n points. */
+int64_t rg_knn_workspace_bytes(int64_t n);
+
+/* distCUDA2 of simple-knn (submodules/simple-knn/simple_knn.cu:186-222, called at
+ * classes/point_cloud.py:206,376,807): out[i] = mean of the squared distances from point i to its
+ * 3 nearest other points (FP32 distances; the reference's bucket search is exact, so is this
+ * one).  points: (n, 3) float32; out: (n) float32.  With fewer than 4 points the missing
+ * neighbours count as FLT_MAX, as in the reference. */
+int rg_knn_mean_dist(const float* points, int64_t n, float* out, void* workspace, int64_t workspace_bytes,
+                     void* stream);
+
+/* One camera of the 3D filter, prepared on the host from the reference's camera:
+ * R (row-major, as camera.R: x_cam = x R + T), T, fx = fov2focal(FoVx, W), fy, cx = W/2, cy = H/2,
+ * and the screen window [-0.15 W, 1.15 W] x [-0.15 H, 1.15 H], all rounded to float. */
+typedef struct rg_filter_camera {
+    float R[9];
+    float T[3];
+    float fx, fy, cx, cy;
+    float x_lo, x_hi, y_lo, y_hi;
+} rg_filter_camera;
+
+/* PointCloud.compute_3D_filter (classes/point_cloud.py:1152-1215): out[i] =
+ * log(0.5 * d_i / focal) with d_i the smallest camera-space depth z > 0.2 over the cameras whose
+ * widened screen window holds the projection (100000 if none is smaller), points seen by no
+ * camera taking the largest d_i of the seen ones.  cams: DEVICE array of num_cams;
+ * inv_focal = float(1 / max over cameras of fx, fy); workspace: device, >= 256 bytes; its first
+ * uint32 holds, after the call, the bits of the largest seen d_i (0 when no point is seen, where
+ * the reference raises: the outputs are then -inf). */
+int rg_filter_3d(const float* points, int64_t n, const rg_filter_camera* cams, int32_t num_cams, float inv_focal,
+                 float* out, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Per-iteration gradient statistics of densification (classes/point_cloud.py:632-640):
+ * accum += |g_i|, num += (|g_i| > 0), and num_gnv += (|g_i| > 0) when num_gnv is not NULL;
+ * grad_positions: (n, 3). */
+int rg_accumulate_grad_norm(const float* grad_positions, int64_t n, float* accum, float* num, float* num_gnv,
+                            void* stream);
+
+/* Inputs of densify_and_prune (classes/point_cloud.py:1067-1143): gradient statistics, the
+ * parameters that decide and transform, and the thresholds.  scale_limit = float(0.01 *
+ * scene_extent); split_n = N children per split Gaussian. */
+typedef struct rg_densify_in {
+    int64_t n;
+    const float* grad_accum;   /* xyz_gradient_accum_norm (n) */
+    const float* num_accum;    /* num_accum (n) */
+    const float* positions;    /* (n, 3) */
+    const float* log_scales;   /* (n, 3) */
+    const float* quats;        /* (n, 4) raw (w, x, y, z) */
+    const float* densities;    /* (n) */
+    float grad_threshold;
+    float scale_limit;
+    float min_density;
+    int32_t split_n;
+} rg_densify_in;
+
+/* One per-Gaussian array to carry through densification: rows of `width` floats.  zero_new = 0
+ * copies a source row to each of its new rows (parameters, filter_3D), 1 gives new rows zeros
+ * (Adam moments, visibility counters). */
+typedef struct rg_densify_array {
+    const float* src;
+    float* dst;
+    int64_t width;
+    int32_t zero_new;
+} rg_densify_array;
+
+int64_t rg_densify_workspace_bytes(int64_t n);
+
+/* First half of densify_and_prune: clone / split / prune decisions and the destination of every
+ * row.  counts (device int64[4]): kept originals, kept clones, split Gaussians, split Gaussians
+ * whose children are kept.  The output has counts[0] + counts[1] + split_n * counts[3] rows:
+ * surviving originals, then clones, then children (child-major), each in source order, as the
+ * reference's cat / boolean-mask sequence leaves them. */
+int rg_densify_plan(const rg_densify_in* in, void* workspace, int64_t workspace_bytes, int64_t* counts,
+                    void* stream);
+
+/* Second half: writes every array's rows to their destinations, then the transformed rows:
+ * halved densities of cloned Gaussians and their clones, and for the children
+ * position = R(q) (s * z) + mu, log-scale = log(exp(log s) / (0.8 N)), density / 2, where
+ * z = normals[c * counts[2] + rank] (the split Gaussian's rank among all split ones, child c)
+ * are standard normal draws.  counts_host: the four counts read back from the plan; the
+ * positions / log_scales / densities arrays must be among `arrays` and their destinations be
+ * out_positions / out_log_scales / out_densities. */
+int rg_densify_apply(const rg_densify_in* in, const void* workspace, int64_t workspace_bytes,
+                     const int64_t* counts_host, const rg_densify_array* arrays, int32_t num_arrays,
+                     const float* normals, float* out_positions, float* out_log_scales, float* out_densities,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ Public surface:
   FusedAdam, step_all, project_params       -- optimiser step + post-step projections (optim.py)
   l1_ssim_loss                              -- the training loss and its gradient (loss.py)
   viewer, formats                           -- RGBA8 + depth frames; RayGauss PLY / checkpoints
+  densify                                   -- distCUDA2, 3D filter, clone/split/prune
   scenes, cameras                           -- synthetic workloads and pinhole cameras
 """
 

@@ -60,6 +60,22 @@ class Projection(C.Structure):
                 ("normalize_quaternion", C.c_int32), ("lobe_axis", C.c_void_p), ("normalize_lobe_axis", C.c_int32)]
 
 
+class FilterCamera(C.Structure):
+    _fields_ = [("R", C.c_float * 9), ("T", C.c_float * 3), ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float),
+                ("cy", C.c_float), ("x_lo", C.c_float), ("x_hi", C.c_float), ("y_lo", C.c_float), ("y_hi", C.c_float)]
+
+
+class DensifyIn(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in ("grad_accum", "num_accum", "positions", "log_scales",
+                                                                "quats", "densities")] + [
+        ("grad_threshold", C.c_float), ("scale_limit", C.c_float), ("min_density", C.c_float),
+        ("split_n", C.c_int32)]
+
+
+class DensifyArray(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("width", C.c_int64), ("zero_new", C.c_int32)]
+
+
 EXPORTS = {
     "rg_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Config)]),
     "rg_destroy": (C.c_int, [C.c_void_p]),
@@ -89,6 +105,16 @@ EXPORTS = {
     "rg_loss_workspace_bytes": (C.c_int64, [C.c_int32, C.c_int32]),
     "rg_loss_l1_ssim": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "rg_knn_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "rg_knn_mean_dist": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "rg_filter_3d": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p,
+                               C.c_int64, C.c_void_p]),
+    "rg_accumulate_grad_norm": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rg_densify_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "rg_densify_plan": (C.c_int, [C.POINTER(DensifyIn), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "rg_densify_apply": (C.c_int, [C.POINTER(DensifyIn), C.c_void_p, C.c_int64, C.POINTER(C.c_int64),
+                                   C.POINTER(DensifyArray), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
 }
 
 

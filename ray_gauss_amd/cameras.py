@@ -61,6 +61,17 @@ class Camera:
     device: str = "cpu"
 
     @property
+    def R(self) -> np.ndarray:
+        """classes/cameras.py convention: the transpose of the world-to-camera rotation
+        (x_cam = x_world R + T)."""
+        return np.asarray(self.R_c2w, np.float64)
+
+    @property
+    def T(self) -> np.ndarray:
+        """World-to-camera translation."""
+        return -np.asarray(self.R_c2w, np.float64).T @ np.asarray(self.eye, np.float64)
+
+    @property
     def world_view_transform(self):
         w2c = np.eye(4, dtype=np.float64)
         w2c[:3, :3] = self.R_c2w.T

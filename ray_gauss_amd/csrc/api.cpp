@@ -49,6 +49,9 @@ int fail_free(int code, const std::string& msg) {
     g_free_error = msg;
     return code;
 }
+
+// Row limit of the handle-free per-Gaussian utilities (int32 row indices inside).
+constexpr int64_t kMaxUtilRows = (int64_t)1 << 30;
 }  // namespace
 
 namespace {
@@ -572,6 +575,97 @@ int rg_colors_to_rgba8(const float* rgb, int64_t n, uint8_t* rgba, void* stream)
     if (n < 0 || (n > 0 && (!rgb || !rgba))) return fail_free(RG_ERR_ARG, "rg_colors_to_rgba8: bad argument");
     const hipError_t e = rg::launch_rgba8(n, rgb, reinterpret_cast<uchar4*>(rgba), static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_colors_to_rgba8: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int64_t rg_knn_workspace_bytes(int64_t n) {
+    if (n <= 0 || n > kMaxUtilRows) return 0;
+    return rg::knn_workspace_bytes(n);
+}
+
+int rg_knn_mean_dist(const float* points, int64_t n, float* out, void* workspace, int64_t workspace_bytes,
+                     void* stream) {
+    if (n < 0 || n > kMaxUtilRows) return fail_free(RG_ERR_ARG, "rg_knn_mean_dist: n out of range");
+    if (n == 0) return RG_OK;
+    if (!points || !out || !workspace) return fail_free(RG_ERR_ARG, "rg_knn_mean_dist: null argument");
+    if (workspace_bytes < rg::knn_workspace_bytes(n))
+        return fail_free(RG_ERR_ARG, "rg_knn_mean_dist: workspace smaller than rg_knn_workspace_bytes()");
+    const hipError_t e = rg::launch_knn_mean_dist(points, n, out, workspace, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_knn_mean_dist: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int rg_filter_3d(const float* points, int64_t n, const rg_filter_camera* cams, int32_t num_cams, float inv_focal,
+                 float* out, void* workspace, int64_t workspace_bytes, void* stream) {
+    if (n < 0 || n > kMaxUtilRows || num_cams < 0) return fail_free(RG_ERR_ARG, "rg_filter_3d: size out of range");
+    if (!workspace || workspace_bytes < 256) return fail_free(RG_ERR_ARG, "rg_filter_3d: workspace < 256 bytes");
+    if (n > 0 && (!points || !out || (num_cams > 0 && !cams)))
+        return fail_free(RG_ERR_ARG, "rg_filter_3d: null argument");
+    const hipError_t e = rg::launch_filter_3d(points, n, cams, num_cams, inv_focal, out,
+                                              static_cast<unsigned int*>(workspace), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_filter_3d: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int rg_accumulate_grad_norm(const float* grad_positions, int64_t n, float* accum, float* num, float* num_gnv,
+                            void* stream) {
+    if (n < 0) return fail_free(RG_ERR_ARG, "rg_accumulate_grad_norm: n < 0");
+    if (n > 0 && (!grad_positions || !accum || !num))
+        return fail_free(RG_ERR_ARG, "rg_accumulate_grad_norm: null argument");
+    const hipError_t e = rg::launch_accum_grad(grad_positions, n, accum, num, num_gnv, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_accumulate_grad_norm: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int64_t rg_densify_workspace_bytes(int64_t n) {
+    if (n < 0 || n > kMaxUtilRows) return 0;
+    return rg::densify_workspace_bytes(n);
+}
+
+static int densify_check(const rg_densify_in* in, int64_t workspace_bytes, const char* who) {
+    if (!in) return fail_free(RG_ERR_ARG, std::string(who) + ": null argument");
+    if (in->n < 0 || in->n > kMaxUtilRows) return fail_free(RG_ERR_ARG, std::string(who) + ": n out of range");
+    if (in->split_n < 1 || in->split_n > 64) return fail_free(RG_ERR_ARG, std::string(who) + ": split_n in [1, 64]");
+    if (in->n > 0 && (!in->grad_accum || !in->num_accum || !in->positions || !in->log_scales || !in->quats ||
+                      !in->densities))
+        return fail_free(RG_ERR_ARG, std::string(who) + ": null input array");
+    if (workspace_bytes < rg::densify_workspace_bytes(in->n))
+        return fail_free(RG_ERR_ARG, std::string(who) + ": workspace smaller than rg_densify_workspace_bytes()");
+    return RG_OK;
+}
+
+int rg_densify_plan(const rg_densify_in* in, void* workspace, int64_t workspace_bytes, int64_t* counts,
+                    void* stream) {
+    int rc;
+    if ((rc = densify_check(in, workspace_bytes, "rg_densify_plan"))) return rc;
+    if (!workspace || !counts) return fail_free(RG_ERR_ARG, "rg_densify_plan: null argument");
+    const hipError_t e = rg::launch_densify_plan(*in, workspace, counts, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_densify_plan: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
+int rg_densify_apply(const rg_densify_in* in, const void* workspace, int64_t workspace_bytes,
+                     const int64_t* counts_host, const rg_densify_array* arrays, int32_t num_arrays,
+                     const float* normals, float* out_positions, float* out_log_scales, float* out_densities,
+                     void* stream) {
+    int rc;
+    if ((rc = densify_check(in, workspace_bytes, "rg_densify_apply"))) return rc;
+    if (!workspace || !counts_host) return fail_free(RG_ERR_ARG, "rg_densify_apply: null argument");
+    if (num_arrays < 0 || num_arrays > rg::densify_max_arrays() || (num_arrays > 0 && !arrays))
+        return fail_free(RG_ERR_ARG, "rg_densify_apply: num_arrays out of range");
+    const int64_t ko = counts_host[0], kc = counts_host[1], ns = counts_host[2], kch = counts_host[3];
+    if (ko < 0 || kc < 0 || ns < 0 || kch < 0 || kch > ns || ko + ns > in->n || kc > in->n)
+        return fail_free(RG_ERR_ARG, "rg_densify_apply: counts inconsistent with n");
+    if (in->n > 0 && (!out_positions || !out_log_scales || !out_densities))
+        return fail_free(RG_ERR_ARG, "rg_densify_apply: null output");
+    if (ns > 0 && !normals) return fail_free(RG_ERR_ARG, "rg_densify_apply: normals needed for split Gaussians");
+    for (int k = 0; k < num_arrays; ++k)
+        if (arrays[k].width <= 0 || (in->n > 0 && (!arrays[k].src || !arrays[k].dst)))
+            return fail_free(RG_ERR_ARG, "rg_densify_apply: bad array entry");
+    const hipError_t e = rg::launch_densify_apply(*in, workspace, counts_host, arrays, num_arrays, normals,
+                                                  out_positions, out_log_scales, out_densities,
+                                                  static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_densify_apply: ") + hipGetErrorString(e));
     return RG_OK;
 }
 

@@ -314,19 +314,27 @@ inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 // Build: expects leafbox in ORIGINAL order (preprocess run with identity iperm) and the scene box.
 // Produces perm/iperm, topology, links.  Boxes are written by a following refit.
+int64_t radix_hist_bytes(int n) { return (int64_t)256 * blocks(n, kTile) * sizeof(unsigned int); }
+
+int launch_radix_sort(int n, int passes, unsigned long long* keys[2], int* vals[2], unsigned int* hist,
+                      hipStream_t s) {
+    const int tiles = blocks(n, kTile);
+    int cur = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = pass * 8;
+        hipLaunchKernelGGL(k_hist, dim3(tiles), dim3(kSortBlock), 0, s, n, shift, keys[cur], hist, tiles);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, hist, 256 * tiles);
+        hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kSortBlock), 0, s, n, shift, keys[cur], vals[cur],
+                           keys[cur ^ 1], vals[cur ^ 1], hist, tiles);
+        cur ^= 1;
+    }
+    return cur;
+}
+
 hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, BvhBuffers& b, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s, n, leafbox, box, b.keys[0], b.vals[0]);
-    const int tiles = blocks(n, kTile);
-    int cur = 0;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = pass * 8;
-        hipLaunchKernelGGL(k_hist, dim3(tiles), dim3(kSortBlock), 0, s, n, shift, b.keys[cur], b.hist, tiles);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, b.hist, 256 * tiles);
-        hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kSortBlock), 0, s, n, shift, b.keys[cur], b.vals[cur],
-                           b.keys[cur ^ 1], b.vals[cur ^ 1], b.hist, tiles);
-        cur ^= 1;
-    }
+    const int cur = launch_radix_sort(n, 8, b.keys, b.vals, b.hist, s);
     hipLaunchKernelGGL(k_set_perm, dim3(blocks(n, 256)), dim3(256), 0, s, n, b.vals[cur], b.perm, b.iperm);
     const int total = 2 * n - 1;
     hipLaunchKernelGGL(k_fill, dim3(blocks(total, 256)), dim3(256), 0, s, b.k_parent, total, -1);

@@ -102,6 +102,11 @@ hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, Bvh
 hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b, hipStream_t s);
 hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s);
 hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s);
+// Stable LSD radix sort of (keys, vals) over the low 8*passes bits; returns the index of the
+// buffer pair holding the result.  hist needs radix_hist_bytes(n).
+int launch_radix_sort(int n, int passes, unsigned long long* keys[2], int* vals[2], unsigned int* hist,
+                      hipStream_t s);
+int64_t radix_hist_bytes(int n);
 
 // ---- launchers (optim.hip) ---------------------------------------------------------------------
 constexpr int kMaxAdamTensors = 16;
@@ -131,6 +136,19 @@ struct AdamBatch {
 };
 hipError_t launch_adam(const AdamBatch& b, int64_t total_items, hipStream_t s);
 hipError_t launch_project(const rg_projection& pr, hipStream_t s);
+
+// ---- launchers (densify.hip) -------------------------------------------------------------------
+int64_t knn_workspace_bytes(int64_t n);
+hipError_t launch_knn_mean_dist(const float* points, int64_t n, float* out, void* ws, hipStream_t s);
+hipError_t launch_filter_3d(const float* points, int64_t n, const rg_filter_camera* cams_dev, int V, float inv_focal,
+                            float* out, unsigned int* dmax_bits, hipStream_t s);
+hipError_t launch_accum_grad(const float* g, int64_t n, float* accum, float* num, float* gnv, hipStream_t s);
+int64_t densify_workspace_bytes(int64_t n);
+hipError_t launch_densify_plan(const rg_densify_in& in, void* ws, int64_t* counts, hipStream_t s);
+hipError_t launch_densify_apply(const rg_densify_in& in, const void* ws, const int64_t* counts,
+                                const rg_densify_array* arrays, int num_arrays, const float* normals, float* out_pos,
+                                float* out_ls, float* out_sig, hipStream_t s);
+int densify_max_arrays();
 
 // ---- launchers (loss.hip) ----------------------------------------------------------------------
 int64_t loss_workspace_bytes(int W, int H);
