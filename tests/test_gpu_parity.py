@@ -105,7 +105,7 @@ def test_backward_parity_small(small):
         assert e <= GRAD_TOL, (k, e, rel_err(g[k].cpu().numpy(), og[k]))
 
 
-@pytest.mark.parametrize("depth,seg,packet", [(8, 1, False), (32, 64, True), (16, 8, False)])
+@pytest.mark.parametrize("depth,seg,packet", [(8, 1, False), (32, 64, True), (16, 8, False), (8, 4, True)])
 def test_tuning_invariance(gpu_device, depth, seg, packet):
     # the reference case uses the 6-entry forward list, the others the backward's depth
     """Hit-buffer depth, segment length and packet traversal change the schedule, not the image:
@@ -119,6 +119,29 @@ def test_tuning_invariance(gpu_device, depth, seg, packet):
     a = rc0.forward(0)[0].cpu().numpy()
     b = rc1.forward(0)[0].cpu().numpy()
     assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+def test_traversal_modes_agree(gpu_device):
+    """Per-lane walks and packet walks reach the same leaves in the same order, including the
+    slabs whose hit sets overflow the list (their per-slab walks and the backward's wave-merged
+    scatter over them): identical images, and gradients equal up to the order of the atomic
+    sums."""
+    wl = scenes.workload("blobs100k")
+    cam = wl.cameras[0].launch_record()
+    px = torch.from_numpy(scenes.crop_pixels(wl.cameras[0], 384, 384, 32, 32))
+    target = torch.from_numpy(np.random.default_rng(5).uniform(0, 1, (len(px), 3)).astype(np.float32)).cuda()
+    out = {}
+    for mode in (False, True):
+        rc, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device, TuneConfig(8, 8, packet=mode, fwd_hit_depth=6))
+        rgb, _, _ = rc.forward(0, px)
+        st = rc.stats()
+        rc.backward(0, px, rgb, 2.0 * (rgb - target))
+        g = {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}
+        out[mode] = (rgb.cpu().numpy(), g, st)
+    assert out[False][2]["fallback_slabs"] > 0  # the direct (overflow) path is exercised
+    assert np.array_equal(out[True][0], out[False][0])
+    for k, v in out[False][1].items():
+        assert max_rel(out[True][1][k], v) <= 1e-5, k
 
 
 def test_blobs_crop_parity(gpu_device):
