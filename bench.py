@@ -152,7 +152,8 @@ def full_step(args):
     rv, rp, _ = make_ray_batch(cams, wl.pixels, dev)
     R = int(rv.shape[0])
     target = torch.rand((R, 3), generator=torch.Generator(device=dev).manual_seed(0), device=dev)
-    tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth)
+    tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
+                      replay_words=args.replay_words)
     trainer = BatchedTrainer(RayCaster(wl.params, tune), buf, cams, rv, rp, target, loss_kind=0,
                              use_graph=not args.no_graph)
     opts = [FusedAdam([raw[k]], lr=FULL_STEP_LR[k]) for k in raw]
@@ -213,6 +214,8 @@ def main():
     ap.add_argument("--hit-depth", type=int, default=8)
     ap.add_argument("--seg-slabs", type=int, default=8)
     ap.add_argument("--fwd-hit-depth", type=int, default=6, help="list depth of the forward kernel (6, 8, 16, 32)")
+    ap.add_argument("--replay-words", type=int, default=1024,
+                    help="words per ray of the forward's hit-list records replayed by the backward (0: off)")
     ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
@@ -268,7 +271,8 @@ def main():
     targets_pv = [target[int(sum(len(p) for p in pixels[:v])):][:len(pixels[v])] for v in range(len(cams))]
     px_dev = [torch.from_numpy(p).to(dev) for p in pixels]
 
-    tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth)
+    tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
+                      replay_words=args.replay_words)
     rc = RayCaster(wl.params, tune)
 
     def prepare(rebuild=False):
@@ -356,11 +360,13 @@ def main():
                    "sizes": sorted({(c["width"], c["height"]) for c in cams})[:4], "rays_per_step": R,
                    "sh_degree": sc.sh_degree, "sg_lobes": L, "dt": wl.params.dt, "t_eps": wl.params.t_eps,
                    "hit_depth": tune.hit_depth, "seg_slabs": tune.seg_slabs,
-                   "packet_traversal": tune.packet, "fwd_hit_depth": tune.fwd_hit_depth},
+                   "packet_traversal": tune.packet, "fwd_hit_depth": tune.fwd_hit_depth,
+                   "replay_words": tune.replay_words},
         "per_view_rays_per_s": (R / dt_pv) if dt_pv else None,
         "batched_over_per_view": (dt_pv / dt_b) if dt_pv else None,
         "counters": {"K_slab_hits": K, "P_pairs": P, "fallback_slabs": st["fallback_slabs"],
-                     "gathers": st["gathers"]},
+                     "gathers": st["gathers"], "replay_words": st["replay_words"],
+                     "replay_truncated_rays": st["replay_truncated"]},
         "roofline": {"bound": "hbm", "kernel": "k_backward", "kernel_ms": bk_ms, "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "traffic_source": traffic_src, "algorithmic_bytes": B_bwd},

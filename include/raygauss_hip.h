@@ -54,6 +54,11 @@ typedef struct rg_config {
     uint32_t iteration;        /* RenderOptixSettings.iteration: seeds tea<4>(iteration, pixel)       */
     int32_t fwd_hit_depth;     /* list capacity of the forward kernel: 6, 8, 16, 32 (0: hit_depth);
                                   a smaller list lets more wavefronts share a CU's LDS            */
+    int32_t replay_words;      /* words per ray, a multiple of 8 (0: off), in which a forward records the hit lists
+                                  it gathered; the next backward over the same ray set (same
+                                  arrays and view, no rg_prepare / rg_set_views / rg_set_config
+                                  in between) reads them instead of walking the LBVH again, and
+                                  rays whose lists did not fit walk the tree for the rest      */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of
@@ -115,6 +120,8 @@ typedef struct rg_stats {
     int64_t overflow_slabs;   /* slabs with more hits than max_hits_per_slab (optix_utils:205) */
     int64_t fallback_slabs;   /* slabs whose hit set exceeded hit_depth (per-slab traversal)   */
     int64_t gathers;          /* interval-list gathers (BVH traversals)                       */
+    int64_t replay_words;     /* words of hit-list records written by the forward (rg_train_batch) */
+    int64_t replay_truncated; /* rays whose records did not fit replay_words                    */
 } rg_stats;
 
 /* Creates a handle.  Replaces create_context/module/pipeline (optix_utils.py:77-110): the

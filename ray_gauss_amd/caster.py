@@ -21,12 +21,15 @@ from .cameras import launch_record_from
 @dataclass
 class TuneConfig:
     """Launch tuning: interval-list depth per ray, initial slabs per gather, whether a wavefront's
-    lanes walk the LBVH as one packet, and the forward kernel's own (smaller) list depth."""
+    lanes walk the LBVH as one packet, the forward kernel's own (smaller) list depth, and the
+    words per ray in which a forward records its hit lists for the backward over the same rays
+    (0: the backward walks the tree again; 1024 words = 4 KB of device memory per ray)."""
 
     hit_depth: int = 8
     seg_slabs: int = 8
     packet: bool = True
     fwd_hit_depth: int = 6
+    replay_words: int = 1024
 
 
 def _ptr(t):
@@ -105,6 +108,7 @@ class RayCaster:
         c.max_hits_per_slab = int(self.max_hits_per_slab)
         c.packet_traversal = int(bool(self.tune.packet))
         c.fwd_hit_depth = int(self.tune.fwd_hit_depth)
+        c.replay_words = int(getattr(self.tune, "replay_words", 0))
         c.jitter = int(getattr(p, "jitter", 0))
         c.rnd_sample = int(getattr(p, "rnd_sample", 0))
         c.iteration = int(getattr(p, "iteration", 0) or 0) & 0xFFFFFFFF

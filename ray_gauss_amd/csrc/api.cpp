@@ -39,6 +39,21 @@ struct rg_handle {
     // colour buffer of rg_train_batch when the caller passes no out_rgb
     float* train_rgb = nullptr;
     int64_t train_rgb_cap = 0;
+    // hit-list records of the last forward (rays x replay_words) and the ray set they belong to
+    unsigned int* rec = nullptr;
+    int* rec_len = nullptr;
+    int64_t rec_cap_words = 0;
+    int64_t rec_cap_rays = 0;
+    struct {
+        bool valid = false;
+        int gen = 0;
+        const void* ray_view = nullptr;
+        const void* ray_pixel = nullptr;
+        const void* pixels = nullptr;
+        int64_t num_rays = 0;
+        int view = 0;
+        int words = 0;
+    } rec_set;
 };
 
 namespace {
@@ -98,6 +113,8 @@ int check_config(rg_handle* h, const rg_config* c) {
         return fail(h, RG_ERR_ARG, "fwd_hit_depth must be 0, 6, 8, 16 or 32");
     if (c->jitter != 0 && c->jitter != 1) return fail(h, RG_ERR_ARG, "jitter must be 0 or 1");
     if (c->rnd_sample < 0 || c->rnd_sample > 2) return fail(h, RG_ERR_ARG, "rnd_sample must be 0, 1 or 2");
+    if (c->replay_words < 0 || c->replay_words > (1 << 20) || (c->replay_words & 7))
+        return fail(h, RG_ERR_ARG, "replay_words must be a multiple of 8 in 0..2^20");
     return RG_OK;
 }
 
@@ -178,6 +195,10 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.jitter = h->cfg.jitter;
     p.rnd_sample = h->cfg.rnd_sample;
     p.iteration = h->cfg.iteration;
+    p.rec = nullptr;
+    p.rec_len = nullptr;
+    p.rec_words = 0;
+    p.rec_mode = 0;
     return p;
 }
 
@@ -255,7 +276,7 @@ int rg_destroy(rg_handle* h) {
                     h->bvh.vals[0], h->bvh.vals[1], h->bvh.hist, h->bvh.perm, h->bvh.iperm, h->bvh.nodes,
                     h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
                     h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb,
-                    h->train_rgb};
+                    h->train_rgb, h->rec, h->rec_len};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete h;
@@ -268,6 +289,7 @@ int rg_set_config(rg_handle* h, const rg_config* cfg) {
     if (rc) return rc;
     const bool thr_changed = cfg->sigma_thr != h->cfg.sigma_thr;
     h->cfg = *cfg;
+    h->rec_set.valid = false;  // recorded hit lists belong to the previous settings
     if (thr_changed) h->built = false;  // records depend on the threshold: prepare again
     return RG_OK;
 }
@@ -316,6 +338,7 @@ int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* s
         if (cams[v].width <= 0 || cams[v].height <= 0) return fail(h, RG_ERR_ARG, "camera size must be positive");
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc;
+    h->rec_set.valid = false;  // recorded hit lists belong to the previous cameras
     if ((rc = ensure_view_capacity(h, num_views))) return rc;
     h->host_cams.assign(cams, cams + num_views);
     HIPCHK(h, hipMemcpyAsync(h->cams, cams, sizeof(rg_camera) * num_views, hipMemcpyHostToDevice, s));
@@ -333,6 +356,48 @@ int rg_set_view_colors(rg_handle* h, int32_t view, const float* rgb, void* strea
     return RG_OK;
 }
 
+// Hit-list records: with cfg.replay_words > 0 a forward records, per ray, the lists it gathered;
+// a backward over the same ray set (same arrays, same view, no rg_prepare, rg_set_views or
+// rg_set_config in between) replays them instead of walking the LBVH again.
+static void rec_forward(rg_handle* h, const rg::RayJob& j, rg::DevParams& p) {
+    h->rec_set.valid = false;
+    const int words = h->cfg.replay_words;
+    if (words <= 0 || j.num_rays <= 0) return;
+    if (h->rec_cap_rays < j.num_rays || h->rec_cap_words < words) {
+        h->rec_cap_rays = 0;
+        h->rec_cap_words = 0;
+        // without the memory the pass runs without records (the backward walks the tree)
+        const int64_t rays = (j.num_rays + 63) & ~(int64_t)63;  // whole groups of 64 rays
+        if (dalloc(h, &h->rec, rays * (int64_t)words) != RG_OK || dalloc(h, &h->rec_len, j.num_rays) != RG_OK)
+            return;
+        h->rec_cap_rays = j.num_rays;
+        h->rec_cap_words = words;
+    }
+    p.rec = h->rec;
+    p.rec_len = h->rec_len;
+    p.rec_words = words;
+    p.rec_mode = 1;
+    h->rec_set.valid = true;
+    h->rec_set.gen = h->generation;
+    h->rec_set.ray_view = j.ray_view;
+    h->rec_set.ray_pixel = j.ray_pixel;
+    h->rec_set.pixels = j.pixels;
+    h->rec_set.num_rays = j.num_rays;
+    h->rec_set.view = j.view;
+    h->rec_set.words = words;
+}
+
+static void rec_backward(const rg_handle* h, const rg::RayJob& j, rg::DevParams& p) {
+    const auto& r = h->rec_set;
+    if (!r.valid || r.gen != h->generation || r.words != h->cfg.replay_words || r.ray_view != j.ray_view ||
+        r.ray_pixel != j.ray_pixel || r.pixels != j.pixels || r.num_rays != j.num_rays || r.view != j.view)
+        return;
+    p.rec = h->rec;
+    p.rec_len = h->rec_len;
+    p.rec_words = r.words;
+    p.rec_mode = 2;
+}
+
 int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, float* out_rgb, float* out_depth,
                float* out_opacity, void* stream) {
     if (!h || !out_rgb) return fail(h, RG_ERR_ARG, "null argument");
@@ -345,8 +410,9 @@ int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixe
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
-    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), job, fwd_depth(h), out_rgb, out_depth, out_opacity,
-                                 dev_stats(h), s));
+    rg::DevParams p = dev_params(h);
+    rec_forward(h, job, p);
+    HIPCHK(h, rg::launch_forward(dev_scene(h), p, job, fwd_depth(h), out_rgb, out_depth, out_opacity, dev_stats(h), s));
     return RG_OK;
 }
 
@@ -362,8 +428,10 @@ int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pix
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
     const rg::DevGrad gr = dev_grad(h);
-    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), job, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1,
-                                  gr, dev_stats(h), s));
+    rg::DevParams p = dev_params(h);
+    rec_backward(h, job, p);
+    HIPCHK(h, rg::launch_backward(dev_scene(h), p, job, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1, gr,
+                                  dev_stats(h), s));
     return RG_OK;
 }
 
@@ -391,8 +459,9 @@ int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float*
     if ((rc = batch_job(h, b, j))) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
-    HIPCHK(h, rg::launch_forward(dev_scene(h), dev_params(h), j, fwd_depth(h), out_rgb, out_depth, out_opacity,
-                                 dev_stats(h), s));
+    rg::DevParams p = dev_params(h);
+    rec_forward(h, j, p);
+    HIPCHK(h, rg::launch_forward(dev_scene(h), p, j, fwd_depth(h), out_rgb, out_depth, out_opacity, dev_stats(h), s));
     return RG_OK;
 }
 
@@ -404,8 +473,10 @@ int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colo
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::DevGrad gr = dev_grad(h);
-    HIPCHK(h, rg::launch_backward(dev_scene(h), dev_params(h), j, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1,
-                                  gr, dev_stats(h), s));
+    rg::DevParams p = dev_params(h);
+    rec_backward(h, j, p);
+    HIPCHK(h, rg::launch_backward(dev_scene(h), p, j, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1, gr,
+                                  dev_stats(h), s));
     return RG_OK;
 }
 
@@ -428,8 +499,10 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
         rgb = h->train_rgb;
     }
     const rg::DevScene sc = dev_scene(h);
-    const rg::DevParams p = dev_params(h);
+    rg::DevParams p = dev_params(h);
+    rec_forward(h, j, p);
     HIPCHK(h, rg::launch_forward(sc, p, j, fwd_depth(h), rgb, nullptr, nullptr, dev_stats(h), s));
+    rec_backward(h, j, p);
     HIPCHK(h, rg::launch_backward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, target, loss_kind, 0, dev_grad(h),
                                   dev_stats(h), s));
     if (out_loss)
@@ -467,7 +540,7 @@ int rg_scene_box(rg_handle* h, float* out6, void* stream) {
 
 int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
     if (!h || !out) return fail(h, RG_ERR_ARG, "null argument");
-    unsigned long long c[8];
+    unsigned long long c[10];
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
@@ -476,6 +549,8 @@ int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
     out->overflow_slabs = (int64_t)c[2];
     out->fallback_slabs = (int64_t)c[3];
     out->gathers = (int64_t)c[4];
+    out->replay_words = (int64_t)c[8];
+    out->replay_truncated = (int64_t)c[9];
     return RG_OK;
 }
 

@@ -202,6 +202,18 @@ __device__ __forceinline__ bool box_hit(const float4 lo, const float4 hi, const 
     return tn <= tf;
 }
 
+// Tree nodes are read through the constant address space: the traversal kernels never write
+// them, and this lets a wave-uniform node index become a scalar load (through the scalar cache)
+// even in kernels that store or add to global memory, where the compiler cannot otherwise prove
+// the node array unchanged and falls back to per-lane vector loads.
+typedef float node_f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const node_f32x4 const_node_f32x4;
+
+__device__ __forceinline__ float4 ld_node(const float4* __restrict__ nodes, int i) {
+    const node_f32x4 v = ((const const_node_f32x4*)nodes)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Stackless traversal of the threaded LBVH: visits, in DFS preorder, every leaf whose padded box
 // meets the ray segment [ta, tb].  on_leaf(slot) returns false to stop the traversal.
 template <typename F>
@@ -210,8 +222,8 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, int t
     const float ix = 1.0f / r.dx, iy = 1.0f / r.dy, iz = 1.0f / r.dz;
     int idx = 0;
     while (idx < total) {
-        const float4 lo = nodes[2 * idx];
-        const float4 hi = nodes[2 * idx + 1];
+        const float4 lo = ld_node(nodes, 2 * idx);
+        const float4 hi = ld_node(nodes, 2 * idx + 1);
         const int miss = __float_as_int(lo.w);
         const int leaf = __float_as_int(hi.w);
         if (miss <= idx) return;  // malformed link: never loop (idx strictly increases)
@@ -243,8 +255,8 @@ __device__ __forceinline__ void traverse_packet(const float4* __restrict__ nodes
     int idx = 0;
     while (idx < total && __ballot(live) != 0ULL) {
         idx = __builtin_amdgcn_readfirstlane(idx);
-        const float4 lo = nodes[2 * idx];
-        const float4 hi = nodes[2 * idx + 1];
+        const float4 lo = ld_node(nodes, 2 * idx);
+        const float4 hi = ld_node(nodes, 2 * idx + 1);
         const int miss = __float_as_int(lo.w);
         const int leaf = __float_as_int(hi.w);
         if (miss <= idx) return;  // malformed link: never loop (idx strictly increases)

@@ -51,6 +51,12 @@ struct DevParams {
     int jitter;      // sub-pixel jitter of the rays
     int rnd_sample;  // sample offsets: 0 none, 1 random, 2 half a sample
     unsigned int iteration;  // seeds each ray's generator: tea4(iteration, pixel)
+    // hit-list records (rg_train_batch): ray r owns words rec[r * rec_words ...]; rec_len[r] = words
+    // written (bit 31: the records stop early).  rec_mode 0 off, 1 forward writes, 2 backward reads.
+    unsigned int* rec;
+    int* rec_len;
+    int rec_words;
+    int rec_mode;
 };
 
 // Gradient accumulators (device-scope atomic adds; the finish kernel reads and clears them).

@@ -40,7 +40,8 @@ constexpr int kWave = 64;
 constexpr int kMaxSeg = 4096;
 constexpr int kGradVals = 14;
 
-enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6 };
+enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6, C_REC_WORDS = 8,
+                  C_REC_FULL = 9 };
 
 struct RayOut {
     float C[3];
@@ -85,7 +86,7 @@ __device__ __forceinline__ Isect entry_isect(const IList<CAP>& L, int e, int lan
 }
 
 struct Counters {
-    unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0;
+    unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0, rec_words = 0, rec_full = 0;
 };
 
 // FP32 gradient add (device scope, relaxed: the result is read by a later kernel).
@@ -359,6 +360,24 @@ __device__ __forceinline__ void trav(const DevParams& p, const DevScene& sc, con
     else traverse(sc.nodes, sc.total_nodes, ray, ta, tb, on_leaf);
 }
 
+// Writes list entry e of the lane from Gaussian `slot` (false when the ray misses its ellipsoid).
+template <int CAP>
+__device__ __forceinline__ bool load_entry(const DevScene& sc, const Ray& ray, int slot, IList<CAP>& L, int e,
+                                           int lane) {
+    float4 g0, g1, g2, g3;
+    load_rec(sc.geo, slot, g0, g1, g2, g3);
+    Isect I;
+    if (!intersect(g0, g1, g2, g3, ray, I)) return false;
+    refine_sampling(g0, g1, g2, g3, ray, I);
+    L.g[e][lane] = slot;
+    L.r1[e][lane] = I.r1;
+    L.r2[e][lane] = I.r2;
+    L.tc[e][lane] = I.tc;
+    L.A[e][lane] = I.A;
+    L.Cq[e][lane] = I.Cq;
+    return true;
+}
+
 // Gathers the intervals overlapping [ta, tb] of the Gaussians with slot > after into the lane's
 // list; returns the count, or CAP + 1 on overflow (the list then holds the first CAP).  Entries
 // come out in DFS order of the tree = increasing Morton slot.
@@ -399,6 +418,7 @@ __device__ __forceinline__ int gather(const DevScene& sc, const DevParams& p, co
 
 struct SegState {
     float tb;       // start of the current slab (reference tbuffer)
+    int k;          // index of that slab along the ray
     float seg_hi;   // end of the gathered segment
     int m;          // segment length in slabs for the next gather
     int cnt;        // entries in the lane's list
@@ -408,6 +428,7 @@ struct SegState {
 
 __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevParams& p) {
     s.tb = ray.tenter;
+    s.k = 0;
     s.seg_hi = ray.tenter;
     s.m = p.seg_slabs;
     s.cnt = 0;
@@ -415,12 +436,118 @@ __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevP
     s.direct = false;
 }
 
+// ------------------------------------------------------------------ hit-list records ---------
+//
+// The forward records, per ray, every list it gathered and every slab it served by per-slab
+// traversal; the backward over the same rays replays them instead of walking the LBVH.  Each
+// record carries the index of the slab where the forward gathered it, and the backward takes
+// every record that starts at or before its current slab, so it holds the forward's list for
+// every slab: the same Gaussians in the same order.  A ray whose records end before its backward
+// does (space ran out, or the backward goes further) continues with its own gathers from there.
+// A record is a whole number of 32-byte sectors written by full 16-byte stores (a partly written
+// sector costs a read-modify-write in the memory):
+//   word 0: bit 31 set, bit 30 direct slab, bits 29..24 list count, bits 21..0 slab index
+//   word 1: end of the gathered segment (float bits), words 2..: the list's slots, zero padded
+constexpr unsigned int kRecOn = 0x80000000u, kRecDirect = 0x40000000u;
+constexpr int kRecMaxSlab = (1 << 22) - 1;
+
+// Layout: the rays of a group of 64 (ray index / 64) share one region; sector c of ray r sits at
+// ((group * C + c) * 64 + r % 64) * 8 words, C = words per ray / 8.  The 64 rays of a wavefront
+// thus write neighbouring sectors (no stride that would send every lane to one memory channel).
+struct Rec {
+    unsigned int* w = nullptr;  // sector 0 of this ray; nullptr: off
+    int pos = 0;                // next word (a multiple of 8)
+    int end = 0;                // capacity (recording) or words recorded (replay)
+    bool full = false;          // recording ran out of space
+};
+
+__device__ __forceinline__ unsigned int* rec_base(unsigned int* rec, int64_t ray, int words) {
+    return rec + (((ray >> 6) * (int64_t)(words >> 3)) * 64 + (ray & 63)) * 8;
+}
+
+__device__ __forceinline__ unsigned int* rec_at(const Rec& rc, int w) { return rc.w + ((w >> 3) << 9) + (w & 7); }
+
+__device__ __forceinline__ int rec_len(int cnt) { return (2 + cnt + 7) & ~7; }
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 template <int CAP>
+__device__ __forceinline__ void record(Rec& rc, int k, bool direct, float hi, const IList<CAP>& L, int cnt,
+                                       int lane) {
+    if (!rc.w || rc.full) return;
+    const int len = rec_len(cnt);
+    if (k > kRecMaxSlab || rc.pos + len > rc.end) {
+        rc.full = true;
+        return;
+    }
+    const unsigned int h0 = kRecOn | (direct ? kRecDirect : 0u) | ((unsigned int)cnt << 24) | (unsigned int)k;
+    for (int c = 0; c < len; c += 4) {
+        unsigned int v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = c + j - 2;
+            v[j] = c + j == 0 ? h0 : c + j == 1 ? __float_as_uint(hi) : (e < cnt ? (unsigned int)L.g[e][lane] : 0u);
+        }
+        // streaming stores: the records are read once, by the next kernel, and must not evict
+        // the tree and Gaussian records the traversal keeps in L2
+        const u32x4 q = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(q, reinterpret_cast<u32x4*>(rec_at(rc, rc.pos + c)));
+    }
+    rc.pos += len;
+}
+
+// Replay: takes every record that starts at or before the current slab.  Anything that does not
+// fit the lane's list or does not reproduce (never expected) ends the replay.
+template <int CAP>
+__device__ __forceinline__ void take_records(SegState& s, const DevScene& sc, const Ray& ray, IList<CAP>& L,
+                                             int lane, Rec& rc) {
+    while (rc.pos + 8 <= rc.end) {
+        const u32x4 h = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(rec_at(rc, rc.pos)));
+        if (!(h.x & kRecOn)) {
+            rc.w = nullptr;
+            s.valid = false;
+            return;
+        }
+        if ((int)(h.x & (unsigned int)kRecMaxSlab) > s.k) return;
+        const int cnt = (int)((h.x >> 24) & 63u);
+        const int len = rec_len(cnt);
+        if (h.x & kRecDirect) {
+            s.direct = true;
+            s.cnt = 0;
+            s.valid = true;
+            rc.pos += len;
+            continue;
+        }
+        bool ok = cnt <= CAP && rc.pos + len <= rc.end;
+        for (int e = 0; ok && e < cnt; ++e) {
+            const int slot = (int)__builtin_nontemporal_load(rec_at(rc, rc.pos + 2 + e));
+            ok = slot >= 0 && slot < sc.n && load_entry<CAP>(sc, ray, slot, L, e, lane);
+        }
+        if (!ok) {
+            rc.w = nullptr;
+            s.valid = false;
+            return;
+        }
+        s.direct = false;
+        s.cnt = cnt;
+        s.seg_hi = __uint_as_float(h.y);
+        s.valid = true;
+        rc.pos += len;
+    }
+}
+
+template <int CAP, int RM>
 __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const DevParams& p, const Ray& ray,
-                                          IList<CAP>& L, int lane, Counters& ctr, float& tmin, float& tmax) {
+                                          IList<CAP>& L, int lane, Counters& ctr, float& tmin, float& tmax, Rec& rc) {
     const float slab = p.dt * (float)p.spb;
     while (true) {
         if (!(s.tb < ray.texit)) return false;
+        if constexpr (RM == 2) {
+            if (rc.w) {
+                take_records<CAP>(s, sc, ray, L, lane, rc);
+                if (!s.valid) rc.w = nullptr;  // no record holds this slab: own gathers from here on
+            }
+        }
         if (!s.valid) {
             s.seg_hi = fminf(ray.texit, s.tb + (float)s.m * slab);
             const int cnt = gather<CAP>(sc, p, ray, s.tb, s.seg_hi, L, lane);
@@ -433,9 +560,11 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
                 ctr.fallback++;
                 s.direct = true;
                 s.cnt = 0;
+                if constexpr (RM == 1) record<CAP>(rc, s.k, true, 0.0f, L, 0, lane);
             } else {
                 s.direct = false;
                 s.cnt = cnt;
+                if constexpr (RM == 1) record<CAP>(rc, s.k, false, s.seg_hi, L, cnt, lane);
             }
             s.valid = true;
         }
@@ -451,6 +580,7 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
                 const float nt = add_exact(s.tb, slab);
                 if (!(nt > s.tb)) return false;
                 s.tb = nt;
+                ++s.k;
                 continue;
             }
         }
@@ -468,9 +598,10 @@ __device__ __forceinline__ bool needs_gather(const SegState& s, const Ray& ray, 
 
 // Wave-synchronous gathers: when one lane executing the call must gather, all of them gather
 // anew from their current slab, so that one packet traversal serves every lane.  The gathered
-// lists hold the same hit sets, so the result does not change.
-__device__ __forceinline__ void sync_gathers(SegState& s, const Ray& ray, const DevParams& p) {
-    if (__ballot(needs_gather(s, ray, p)) != 0ULL) s.valid = false;
+// lists hold the same hit sets, so the result does not change.  Lanes replaying recorded lists
+// (own = false) neither gather nor ask others to.
+__device__ __forceinline__ void sync_gathers(SegState& s, const Ray& ray, const DevParams& p, bool own = true) {
+    if (__ballot(own && needs_gather(s, ray, p)) != 0ULL && own) s.valid = false;
 }
 
 __device__ __forceinline__ bool advance_slab(SegState& s, const DevParams& p) {
@@ -481,6 +612,7 @@ __device__ __forceinline__ bool advance_slab(SegState& s, const DevParams& p) {
     }
     if (!(nt > s.tb)) return false;  // slab below half an ulp of t: no progress possible
     s.tb = nt;
+    ++s.k;
     return true;
 }
 
@@ -529,25 +661,27 @@ __device__ __forceinline__ bool tile_pixel(int64_t w, int width, int height, int
 
 __device__ __forceinline__ void flush_counters(const Counters& c, DevStats st) {
     // wave-level reduction, one atomic per counter per wave
-    unsigned int v[5] = {c.K, c.P, c.ovf, c.fallback, c.gathers};
+    unsigned int v[7] = {c.K, c.P, c.ovf, c.fallback, c.gathers, c.rec_words, c.rec_full};
+    const int idx[7] = {C_K, C_P, C_OVF, C_FALLBACK, C_GATHER, C_REC_WORDS, C_REC_FULL};
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 7; ++k) {
         unsigned int x = v[k];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         v[k] = x;
     }
     if (wave_lane() == 0) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k)
-            if (v[k]) atomicAdd(&st.c[k], (unsigned long long)v[k]);
+        for (int k = 0; k < 7; ++k)
+            if (v[k]) atomicAdd(&st.c[idx[k]], (unsigned long long)v[k]);
     }
 }
 
 // ------------------------------------------------------------------ per-ray bodies -----------
 
-template <int SPB, int CAP, bool RNG>
+template <int SPB, int CAP, bool RNG, int RM>
 __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& ray, unsigned int seed,
-                            const float4* __restrict__ rgbv, IList<CAP>& L, int lane, Counters& ctr, RayOut& o) {
+                            const float4* __restrict__ rgbv, IList<CAP>& L, int lane, Counters& ctr, RayOut& o,
+                            Rec& rc) {
     o.C[0] = o.C[1] = o.C[2] = 0.0f;
     o.depth = 0.0f;
     o.T = 1.0f;
@@ -559,7 +693,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
     while (o.T > p.t_eps) {
         sync_gathers(s, ray, p);
         float tmin, tmax;
-        if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) break;
+        if (!next_slab<CAP, RM>(s, sc, p, ray, L, lane, ctr, tmin, tmax, rc)) break;
         const float tb = s.tb;
         float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
 #pragma unroll
@@ -616,8 +750,8 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
     int idx = 0;
     while (idx < total && __ballot(live) != 0ULL) {
         idx = __builtin_amdgcn_readfirstlane(idx);
-        const float4 lo = sc.nodes[2 * idx];
-        const float4 hi = sc.nodes[2 * idx + 1];
+        const float4 lo = ld_node(sc.nodes, 2 * idx);
+        const float4 hi = ld_node(sc.nodes, 2 * idx + 1);
         const int miss = __float_as_int(lo.w);
         const int leaf = __float_as_int(hi.w);
         if (miss <= idx) break;  // malformed link (uniform): never loop
@@ -664,10 +798,10 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
 // (recomputed buffers, per-sample coefficients, T and Crem advanced); then the wave walks the
 // union of the prepared slabs' hits in increasing (view, slot) order and scatters one summed
 // contribution per Gaussian.  Slabs served by per-slab traversal scatter per lane.
-template <int SPB, int CAP, bool RNG>
+template <int SPB, int CAP, bool RNG, int RM>
 __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray, unsigned int seed,
                               const float4* __restrict__ rgbv, const float* Cfwd, const float* dL, IList<CAP>& L,
-                              GradStage& G, int lane, const DevGrad& gr, int view, Counters& ctr) {
+                              GradStage& G, int lane, const DevGrad& gr, int view, Counters& ctr, Rec& rc) {
     int nst = 0;  // records in the staging area
     bool alive = active && (ray.tenter < ray.texit) && (add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter);
     float T = 1.0f;
@@ -691,8 +825,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
             if (!(T > p.t_eps)) {
                 alive = false;
             } else {
-                sync_gathers(s, ray, p);
-                if (!next_slab<CAP>(s, sc, p, ray, L, lane, ctr, tmin, tmax)) alive = false;
+                sync_gathers(s, ray, p, rc.w == nullptr);
+                if (!next_slab<CAP, RM>(s, sc, p, ray, L, lane, ctr, tmin, tmax, rc)) alive = false;
                 else pend = true;
             }
         }
@@ -812,7 +946,7 @@ __device__ __forceinline__ int64_t block_item(int64_t b, int64_t nblocks) {
     return xcd * per + k;
 }
 
-template <int SPB, int CAP, bool RNG>
+template <int SPB, int CAP, bool RNG, int RM>
 __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
                                                    float* __restrict__ out_rgb, float* __restrict__ out_depth,
                                                    float* __restrict__ out_opac, DevStats st) {
@@ -827,7 +961,17 @@ __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, Ray
         unsigned int seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
         const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
         RayOut o;
-        forward_ray<SPB, CAP, RNG>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o);
+        Rec rc;
+        if constexpr (RM == 1) {
+            rc.w = rec_base(p.rec, q.out, p.rec_words);
+            rc.end = p.rec_words;
+        }
+        forward_ray<SPB, CAP, RNG, RM>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o, rc);
+        if constexpr (RM == 1) {
+            p.rec_len[q.out] = rc.pos | (rc.full ? (int)0x80000000u : 0);
+            ctr.rec_words += rc.pos;
+            ctr.rec_full += rc.full ? 1u : 0u;
+        }
         out_rgb[3 * q.out + 0] = o.C[0] + o.T * p.bg;
         out_rgb[3 * q.out + 1] = o.C[1] + o.T * p.bg;
         out_rgb[3 * q.out + 2] = o.C[2] + o.T * p.bg;
@@ -841,7 +985,7 @@ __global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, Ray
 // per-pixel loss of the rendered colours against target (loss_kind 0: L2 sum (C-t)^2, 1: L1
 // sum |C-t|), whose value is summed into the loss counter.  count_stats = 0 leaves the
 // traversal counters to the forward of the same step.
-template <int SPB, int CAP, bool RNG>
+template <int SPB, int CAP, bool RNG, int RM>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) k_backward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
                                                     const float* __restrict__ ray_colors, const float* __restrict__ dLdC,
                                                     const float* __restrict__ target, int loss_kind, int count_stats,
@@ -859,6 +1003,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
     float dL[3] = {0.0f, 0.0f, 0.0f}, Cf[3] = {0.0f, 0.0f, 0.0f};
     float loss = 0.0f;
     int view = 0;
+    Rec rc;
     if (item < num_items && resolve(job, item, q)) {
         Cf[0] = ray_colors[3 * q.out];
         Cf[1] = ray_colors[3 * q.out + 1];
@@ -883,13 +1028,17 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
         if (dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f) {
             active = true;
             view = q.view;
+            if constexpr (RM == 2) {
+                rc.w = rec_base(p.rec, q.out, p.rec_words);
+                rc.end = min(p.rec_len[q.out] & 0x7fffffff, p.rec_words);
+            }
             const rg_camera cam = job.cams[q.view];
             seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
             ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
         }
     }
-    backward_wave<SPB, CAP, RNG>(sc, p, active, ray, seed, sc.rgb + (size_t)view * sc.n, Cf, dL, L, G, lane, gr, view,
-                            ctr);
+    backward_wave<SPB, CAP, RNG, RM>(sc, p, active, ray, seed, sc.rgb + (size_t)view * sc.n, Cf, dL, L, G, lane, gr, view,
+                            ctr, rc);
     if (count_stats) flush_counters(ctr, st);
     if (target) {
         const float ls = wave_sum64(loss);
@@ -952,31 +1101,44 @@ __global__ void k_reset_counters(unsigned long long* c) {
 }  // namespace
 
 // In the implicit single-view mode RayJob.num_rays carries the padded 8x8-tile item count.
-#define RG_DISPATCH_CAP(SPBC, RNGC, CAPV, KERNEL, ...)                                                            \
+#define RG_DISPATCH_CAP(SPBC, RNGC, RMC, CAPV, KERNEL, ...)                                                       \
     do {                                                                                                             \
-        if (CAPV <= 8) hipLaunchKernelGGL((KERNEL<SPBC, 8, RNGC>), __VA_ARGS__);                                     \
-        else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<SPBC, 16, RNGC>), __VA_ARGS__);                              \
-        else hipLaunchKernelGGL((KERNEL<SPBC, 32, RNGC>), __VA_ARGS__);                                              \
+        if (CAPV <= 8) hipLaunchKernelGGL((KERNEL<SPBC, 8, RNGC, RMC>), __VA_ARGS__);                                \
+        else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<SPBC, 16, RNGC, RMC>), __VA_ARGS__);                         \
+        else hipLaunchKernelGGL((KERNEL<SPBC, 32, RNGC, RMC>), __VA_ARGS__);                                         \
     } while (0)
-#define RG_DISPATCH_CAP6(SPBC, RNGC, CAPV, KERNEL, ...)                                                           \
+#define RG_DISPATCH_CAP6(SPBC, RNGC, RMC, CAPV, KERNEL, ...)                                                      \
     do {                                                                                                             \
-        if (CAPV <= 6) hipLaunchKernelGGL((KERNEL<SPBC, 6, RNGC>), __VA_ARGS__);                                     \
-        else RG_DISPATCH_CAP(SPBC, RNGC, CAPV, KERNEL, __VA_ARGS__);                                                 \
+        if (CAPV <= 6) hipLaunchKernelGGL((KERNEL<SPBC, 6, RNGC, RMC>), __VA_ARGS__);                                \
+        else RG_DISPATCH_CAP(SPBC, RNGC, RMC, CAPV, KERNEL, __VA_ARGS__);                                            \
     } while (0)
-// Kernels are specialised on SPB (8 or 16 samples per slab), the list capacity and on whether
-// the random generator is used at all (jitter or rnd_sample), which costs registers.
-#define RG_DISPATCH_ANY(CAPMACRO, SPBV, CAPV, RNGV, KERNEL, ...)                                                     \
+// Kernels are specialised on SPB (8 or 16 samples per slab), the list capacity, on whether the
+// random generator is used at all (jitter or rnd_sample), which costs registers, and on the
+// hit-list record mode (RMV: 0 none, or the kernel's recording / replaying mode), so that kernels
+// without records carry none of their code.
+#define RG_DISPATCH_ANY(CAPMACRO, SPBV, CAPV, RNGV, RMV, RMON, KERNEL, ...)                                         \
     do {                                                                                                             \
-        if (SPBV == 8) {                                                                                             \
-            if (RNGV) CAPMACRO(8, true, CAPV, KERNEL, __VA_ARGS__);                                                  \
-            else CAPMACRO(8, false, CAPV, KERNEL, __VA_ARGS__);                                                      \
+        if (RMV) {                                                                                                   \
+            if (SPBV == 8) {                                                                                         \
+                if (RNGV) CAPMACRO(8, true, RMON, CAPV, KERNEL, __VA_ARGS__);                                        \
+                else CAPMACRO(8, false, RMON, CAPV, KERNEL, __VA_ARGS__);                                            \
+            } else {                                                                                                 \
+                if (RNGV) CAPMACRO(16, true, RMON, CAPV, KERNEL, __VA_ARGS__);                                       \
+                else CAPMACRO(16, false, RMON, CAPV, KERNEL, __VA_ARGS__);                                           \
+            }                                                                                                        \
         } else {                                                                                                     \
-            if (RNGV) CAPMACRO(16, true, CAPV, KERNEL, __VA_ARGS__);                                                 \
-            else CAPMACRO(16, false, CAPV, KERNEL, __VA_ARGS__);                                                     \
+            if (SPBV == 8) {                                                                                         \
+                if (RNGV) CAPMACRO(8, true, 0, CAPV, KERNEL, __VA_ARGS__);                                           \
+                else CAPMACRO(8, false, 0, CAPV, KERNEL, __VA_ARGS__);                                               \
+            } else {                                                                                                 \
+                if (RNGV) CAPMACRO(16, true, 0, CAPV, KERNEL, __VA_ARGS__);                                          \
+                else CAPMACRO(16, false, 0, CAPV, KERNEL, __VA_ARGS__);                                              \
+            }                                                                                                        \
         }                                                                                                            \
     } while (0)
-#define RG_DISPATCH(...) RG_DISPATCH_ANY(RG_DISPATCH_CAP, __VA_ARGS__)
-#define RG_DISPATCH_FWD(...) RG_DISPATCH_ANY(RG_DISPATCH_CAP6, __VA_ARGS__)
+#define RG_DISPATCH(SPBV, CAPV, RNGV, RMV, ...) RG_DISPATCH_ANY(RG_DISPATCH_CAP, SPBV, CAPV, RNGV, RMV, 2, __VA_ARGS__)
+#define RG_DISPATCH_FWD(SPBV, CAPV, RNGV, RMV, ...) \
+    RG_DISPATCH_ANY(RG_DISPATCH_CAP6, SPBV, CAPV, RNGV, RMV, 1, __VA_ARGS__)
 
 hipError_t launch_reset_counters(DevStats st, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, st.c);
@@ -990,7 +1152,7 @@ hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& 
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
     const bool rng = p.jitter || p.rnd_sample;
-    RG_DISPATCH_FWD(p.spb <= 8 ? 8 : 16, hit_depth, rng, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
+    RG_DISPATCH_FWD(p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 1, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
                 items, out_rgb, out_depth, out_opacity, st);
     return hipGetLastError();
 }
@@ -1003,7 +1165,7 @@ hipError_t launch_backward(const DevScene& sc, const DevParams& p, const RayJob&
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
     const bool rng = p.jitter || p.rnd_sample;
-    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, rng, k_backward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
+    RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 2, k_backward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
                 items, ray_colors, dLdC, target, loss_kind, count_stats, gr, st);
     return hipGetLastError();
 }

@@ -80,6 +80,61 @@ def test_fused_train_equals_forward_backward(blobs):
         assert max_rel(g_fused[k].cpu().numpy(), g_sep[k].cpu().numpy()) <= 1e-5, k
 
 
+@pytest.mark.parametrize("words", [0, 16, 64, 4096])
+def test_replayed_hit_lists_give_the_same_step(blobs, gpu_device, words):
+    """train_batch's backward reads the hit lists its forward recorded (words per ray); rays whose
+    records run out continue with their own traversal, and without records the backward walks
+    the tree: the image and loss are identical, the gradients equal up to the order of atomic
+    sums."""
+    sc, prm, cams, pixels, rc0, t0, rv, rp, offs = blobs
+    gen = torch.Generator(device=gpu_device).manual_seed(2)
+    target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
+    res = {}
+    for w in (0, words):
+        rc, _ = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(replay_words=w))
+        out = torch.empty((rv.shape[0], 3), device=gpu_device)
+        loss = torch.zeros(1, device=gpu_device)
+        rc.train_batch(rv, rp, target, 0, out, loss)
+        st = rc.stats()
+        res[w] = (out.cpu().numpy(), float(loss.item()), {k: v.cpu().numpy() for k, v in rc.finish_grads().items()
+                                                          if v is not None}, st)
+    a, b = res[0], res[words]
+    assert np.array_equal(a[0], b[0]) and abs(a[1] - b[1]) <= 1e-6 * abs(a[1])
+    for k, v in a[2].items():
+        assert max_rel(b[2][k], v) <= 1e-5, k
+    if words:
+        assert b[3]["replay_words"] > 0
+        if words == 16:
+            assert b[3]["replay_truncated"] > 0
+        if words == 4096:
+            assert b[3]["replay_truncated"] == 0
+    else:
+        assert a[3]["replay_words"] == 0
+
+
+def test_replay_only_for_the_recorded_rays(blobs, gpu_device):
+    """forward_batch records, backward_batch replays only over the same ray arrays with nothing
+    changed in between: a backward over other rays, or after set_views, walks the tree and gives
+    the same gradients as without records."""
+    sc, prm, cams, pixels, rc0, t0, rv, rp, offs = blobs
+    gen = torch.Generator(device=gpu_device).manual_seed(3)
+    target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
+    rv2, rp2 = rv.flip(0).contiguous(), rp.flip(0).contiguous()
+
+    def run(words, between):
+        rc, _ = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(replay_words=words))
+        rgb, _, _ = rc.forward_batch(rv, rp)
+        between(rc)
+        rc.backward_batch(rv, rp, rgb, 2.0 * (rgb - target))
+        return {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}
+
+    ref = run(0, lambda rc: None)
+    for between in (lambda rc: None, lambda rc: rc.forward_batch(rv2, rp2), lambda rc: rc.set_views(cams)):
+        g = run(512, between)
+        for k, v in ref.items():
+            assert max_rel(g[k], v) <= 1e-5, k
+
+
 def test_batch_backward_parity_with_oracle(blobs):
     sc, prm, cams, pixels, rc, t, rv, rp, offs = blobs
     rgb, _, _ = rc.forward_batch(rv, rp)
