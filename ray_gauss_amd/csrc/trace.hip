@@ -424,6 +424,8 @@ struct SegState {
     int cnt;        // entries in the lane's list
     bool valid;     // list holds the current segment
     bool direct;    // current slab is processed by per-slab traversal
+    int dpos;       // replay of a direct slab: first word of its recorded hit list
+    int dcnt;       //   and its length (-1: no recorded list, traverse)
 };
 
 __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevParams& p) {
@@ -434,6 +436,8 @@ __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevP
     s.cnt = 0;
     s.valid = false;
     s.direct = false;
+    s.dpos = 0;
+    s.dcnt = -1;
 }
 
 // ------------------------------------------------------------------ hit-list records ---------
@@ -447,9 +451,12 @@ __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevP
 // A record is a whole number of 32-byte sectors written by full 16-byte stores (a partly written
 // sector costs a read-modify-write in the memory):
 //   word 0: bit 31 set, bit 30 direct slab, bits 29..24 list count, bits 21..0 slab index
-//   word 1: end of the gathered segment (float bits), words 2..: the list's slots, zero padded
-constexpr unsigned int kRecOn = 0x80000000u, kRecDirect = 0x40000000u;
+//   word 1: list record: end of the gathered segment (float bits);
+//           direct slab: number of hits recorded after it (kRecNoList: none, the slab is walked)
+//   words 2..: the list's slots, or the direct slab's hits in traversal order, zero padded
+constexpr unsigned int kRecOn = 0x80000000u, kRecDirect = 0x40000000u, kRecNoList = 0xffffffffu;
 constexpr int kRecMaxSlab = (1 << 22) - 1;
+constexpr int kRecMaxDirect = 256;  // longest recorded hit list of a direct slab
 
 // Layout: the rays of a group of 64 (ray index / 64) share one region; sector c of ray r sits at
 // ((group * C + c) * 64 + r % 64) * 8 words, C = words per ray / 8.  The 64 rays of a wavefront
@@ -496,6 +503,31 @@ __device__ __forceinline__ void record(Rec& rc, int k, bool direct, float hi, co
     rc.pos += len;
 }
 
+// Direct slab (forward): the record is opened before the slab's traversal, its hits appended as
+// the traversal reaches them, and closed with their count (a list that does not fit is dropped:
+// the backward then walks that slab).
+__device__ __forceinline__ int rec_direct_open(Rec& rc) {
+    if (!rc.w || rc.full) return -1;
+    if (rc.pos + 8 > rc.end) {
+        rc.full = true;
+        return -1;
+    }
+    return rc.pos;
+}
+
+__device__ __forceinline__ void rec_direct_push(const Rec& rc, int start, int n, int slot) {
+    const int w = start + 2 + n;
+    if (start >= 0 && n < kRecMaxDirect && w < rc.end) *rec_at(rc, w) = (unsigned int)slot;
+}
+
+__device__ __forceinline__ void rec_direct_close(Rec& rc, int start, int n, int k) {
+    if (start < 0) return;
+    const bool listed = n <= kRecMaxDirect && start + rec_len(n) <= rc.end && k <= kRecMaxSlab;
+    *rec_at(rc, start) = kRecOn | kRecDirect | (unsigned int)min(k, kRecMaxSlab);
+    *rec_at(rc, start + 1) = listed ? (unsigned int)n : kRecNoList;
+    rc.pos = start + (listed ? rec_len(n) : 8);
+}
+
 // Replay: takes every record that starts at or before the current slab.  Anything that does not
 // fit the lane's list or does not reproduce (never expected) ends the replay.
 template <int CAP>
@@ -515,7 +547,15 @@ __device__ __forceinline__ void take_records(SegState& s, const DevScene& sc, co
             s.direct = true;
             s.cnt = 0;
             s.valid = true;
-            rc.pos += len;
+            const unsigned int n = h.y;
+            if (n != kRecNoList && n <= (unsigned int)kRecMaxDirect && rc.pos + rec_len((int)n) <= rc.end) {
+                s.dpos = rc.pos + 2;
+                s.dcnt = (int)n;
+                rc.pos += rec_len((int)n);
+            } else {
+                s.dcnt = -1;
+                rc.pos += 8;
+            }
             continue;
         }
         bool ok = cnt <= CAP && rc.pos + len <= rc.end;
@@ -560,7 +600,7 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
                 ctr.fallback++;
                 s.direct = true;
                 s.cnt = 0;
-                if constexpr (RM == 1) record<CAP>(rc, s.k, true, 0.0f, L, 0, lane);
+                s.dcnt = -1;  // recorded (with its hits) by the forward's traversal of the slab
             } else {
                 s.direct = false;
                 s.cnt = cnt;
@@ -705,6 +745,8 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
             ++nh;
         };
         if (s.direct) {
+            int rstart = -1;
+            if constexpr (RM == 1) rstart = rec_direct_open(rc);
             trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
                 float4 g0, g1, g2, g3;
                 load_rec(sc.geo, slot, g0, g1, g2, g3);
@@ -712,9 +754,11 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
                 if (!intersect(g0, g1, g2, g3, ray, I)) return true;
                 if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
                 refine_sampling(g0, g1, g2, g3, ray, I);
+                if constexpr (RM == 1) rec_direct_push(rc, rstart, nh, slot);
                 eval_hit(slot, I, g0.w);
                 return true;
             });
+            if constexpr (RM == 1) rec_direct_close(rc, rstart, nh, s.k);
         } else {
             unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
             while (m != 0u) {
@@ -818,6 +862,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
         unsigned int cur_seed = 0u;
         int cur_cnt = 0;
+        bool cur_dlist = false;  // the entries are a direct slab's recorded hits (all in the slab)
+        int cur_dpos = 0;
         // the prepared slab's sample buffers; after slab_coeffs sig, cr, cg hold the per-sample
         // backward coefficients A_i, Bq_i, E_i read by the scatter below
         float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
@@ -840,18 +886,24 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
 #pragma unroll
                 for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
                 int nh = 0;
-                if (s.direct) {
-                    trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
-                        float4 g0, g1, g2, g3;
-                        load_rec(sc.geo, slot, g0, g1, g2, g3);
-                        Isect I;
-                        if (!intersect(g0, g1, g2, g3, ray, I)) return true;
-                        if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
-                        refine_sampling(g0, g1, g2, g3, ray, I);
-                        accum_hit<SPB, RNG>(I, g0.w, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
-                        ++nh;
-                        return true;
-                    });
+                auto direct_hit = [&](int slot) -> bool {
+                    float4 g0, g1, g2, g3;
+                    load_rec(sc.geo, slot, g0, g1, g2, g3);
+                    Isect I;
+                    if (!intersect(g0, g1, g2, g3, ray, I)) return true;
+                    if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+                    refine_sampling(g0, g1, g2, g3, ray, I);
+                    accum_hit<SPB, RNG>(I, g0.w, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
+                    ++nh;
+                    return true;
+                };
+                if (s.direct && RM == 2 && s.dcnt >= 0) {  // the forward's hits of this slab
+                    for (int i = 0; i < s.dcnt; ++i) {
+                        const int slot = (int)*rec_at(rc, s.dpos + i);
+                        if ((unsigned)slot < (unsigned)sc.n) direct_hit(slot);
+                    }
+                } else if (s.direct) {
+                    trav(p, sc, ray, tmin, tmax, direct_hit);
                 } else {
                     unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
                     while (m != 0u) {
@@ -870,7 +922,12 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     cur_tmin = tmin;
                     cur_tmax = tmax;
                     if (RNG && p.rnd_sample == 1) seed = skip_draws(seed, p.spb);  // the slab's draws are spent
-                    if (s.direct) {
+                    if (s.direct && RM == 2 && s.dcnt >= 0) {
+                        has = true;
+                        cur_dlist = true;
+                        cur_dpos = s.dpos;
+                        cur_cnt = s.dcnt;
+                    } else if (s.direct) {
                         dsc = true;
                     } else {
                         has = true;
@@ -888,7 +945,16 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         int e = 0;
         while (true) {
             int my = INT_MAX;
-            if (has) {
+            if (has && cur_dlist) {
+                while (e < cur_cnt) {
+                    const int sl = (int)*rec_at(rc, cur_dpos + e);
+                    if ((unsigned)sl < (unsigned)sc.n) {
+                        my = key_base + sl;
+                        break;
+                    }
+                    ++e;
+                }
+            } else if (has) {
                 while (e < cur_cnt) {
                     if (overlaps(cur_tmin, cur_tmax, L.r1[e][lane], L.r2[e][lane])) {
                         my = key_base + L.g[e][lane];
@@ -904,13 +970,19 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
 #pragma unroll
             for (int k = 0; k < kGradVals; ++k) v[k] = 0.0f;
             if (act) {
-                const int slot = L.g[e][lane];
+                const int slot = mn - key_base;
                 float4 g0, g1, g2, g3;
                 load_rec(sc.geo, slot, g0, g1, g2, g3);
                 Isect I;
-                intersect(g0, g1, g2, g3, ray, I);
-                refine_sampling(g0, g1, g2, g3, ray, I);
-                grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc, v);
+                // a recorded direct hit reproduces the forward's test (a record that does not is
+                // skipped rather than scattered)
+                const bool ok = intersect(g0, g1, g2, g3, ray, I) &&
+                                (!cur_dlist || overlaps(cur_tmin, cur_tmax, I.r1, I.r2));
+                if (ok) {
+                    refine_sampling(g0, g1, g2, g3, ray, I);
+                    grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc,
+                                        v);
+                }
                 ++e;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
