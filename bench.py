@@ -214,7 +214,7 @@ def main():
     ap.add_argument("--hit-depth", type=int, default=8)
     ap.add_argument("--seg-slabs", type=int, default=8)
     ap.add_argument("--fwd-hit-depth", type=int, default=6, help="list depth of the forward kernel (6, 8, 16, 32)")
-    ap.add_argument("--replay-words", type=int, default=1024,
+    ap.add_argument("--replay-words", type=int, default=1536,
                     help="words per ray of the forward's hit-list records replayed by the backward (0: off)")
     ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
     ap.add_argument("--calibrate-copy", action="store_true",

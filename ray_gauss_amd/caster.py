@@ -23,13 +23,13 @@ class TuneConfig:
     """Launch tuning: interval-list depth per ray, initial slabs per gather, whether a wavefront's
     lanes walk the LBVH as one packet, the forward kernel's own (smaller) list depth, and the
     words per ray in which a forward records its hit lists for the backward over the same rays
-    (0: the backward walks the tree again; 1024 words = 4 KB of device memory per ray)."""
+    (0: the backward walks the tree again; 1536 words = 6 KB of device memory per ray)."""
 
     hit_depth: int = 8
     seg_slabs: int = 8
     packet: bool = True
     fwd_hit_depth: int = 6
-    replay_words: int = 1024
+    replay_words: int = 1536
 
 
 def _ptr(t):
