@@ -426,6 +426,7 @@ struct SegState {
     bool direct;    // current slab is processed by per-slab traversal
     int dpos;       // replay of a direct slab: first word of its recorded hit list
     int dcnt;       //   and its length (-1: no recorded list, traverse)
+    bool dense;     // the last direct slab held more hits than the list: walk the next one too
 };
 
 __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevParams& p) {
@@ -438,6 +439,7 @@ __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevP
     s.direct = false;
     s.dpos = 0;
     s.dcnt = -1;
+    s.dense = false;
 }
 
 // ------------------------------------------------------------------ hit-list records ---------
@@ -587,6 +589,15 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
                 take_records<CAP>(s, sc, ray, L, lane, rc);
                 if (!s.valid) rc.w = nullptr;  // no record holds this slab: own gathers from here on
             }
+        }
+        if (!s.valid && s.dense && s.m == 1) {
+            // inside a dense stretch (the last slab needed its own walk): a one-slab gather would
+            // most likely overflow too, so the slab is walked without trying it
+            ctr.fallback++;
+            s.direct = true;
+            s.cnt = 0;
+            s.dcnt = -1;
+            s.valid = true;
         }
         if (!s.valid) {
             s.seg_hi = fminf(ray.texit, s.tb + (float)s.m * slab);
@@ -759,6 +770,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
                 return true;
             });
             if constexpr (RM == 1) rec_direct_close(rc, rstart, nh, s.k);
+            s.dense = nh > CAP;
         } else {
             unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
             while (m != 0u) {
@@ -902,8 +914,10 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         const int slot = (int)*rec_at(rc, s.dpos + i);
                         if ((unsigned)slot < (unsigned)sc.n) direct_hit(slot);
                     }
+                    s.dense = nh > CAP;
                 } else if (s.direct) {
                     trav(p, sc, ray, tmin, tmax, direct_hit);
+                    s.dense = nh > CAP;
                 } else {
                     unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
                     while (m != 0u) {
