@@ -590,9 +590,9 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
                 if (!s.valid) rc.w = nullptr;  // no record holds this slab: own gathers from here on
             }
         }
-        if (!s.valid && s.dense && s.m == 1) {
-            // inside a dense stretch (the last slab needed its own walk): a one-slab gather would
-            // most likely overflow too, so the slab is walked without trying it
+        if (!s.valid && s.dense) {
+            // inside a dense stretch (the last slab needed its own walk): a gather would most
+            // likely overflow too, so the slab is walked without trying it
             ctr.fallback++;
             s.direct = true;
             s.cnt = 0;
@@ -604,7 +604,9 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
             const int cnt = gather<CAP>(sc, p, ray, s.tb, s.seg_hi, L, lane);
             ctr.gathers++;
             if (cnt > CAP) {
-                if (s.m > 1) {
+                // shorter segments down to two slabs; a single slab is walked rather than
+                // gathered (one traversal either way, and the walk needs no list)
+                if (s.m > 2) {
                     s.m >>= 1;
                     continue;
                 }
