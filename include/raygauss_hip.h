@@ -52,8 +52,9 @@ typedef struct rg_config {
     int32_t jitter;            /* RenderOptixSettings.jitter: sub-pixel jitter (gaussians_aabb.cu:409) */
     int32_t rnd_sample;        /* RenderOptixSettings.rnd_sample: 0 grid, 1 random, 2 mid-sample     */
     uint32_t iteration;        /* RenderOptixSettings.iteration: seeds tea<4>(iteration, pixel)       */
-    int32_t fwd_hit_depth;     /* list capacity of the forward kernel: 6, 8, 16, 32 (0: hit_depth);
-                                  a smaller list lets more wavefronts share a CU's LDS            */
+    int32_t fwd_hit_depth;     /* list capacity of the forward kernel: 6, 8 (as 12), 12, 16, 32
+                                  (0: hit_depth); a smaller list lets more wavefronts share a CU's
+                                  LDS (12 bytes per entry and lane)                             */
     int32_t replay_words;      /* words per ray, a multiple of 8 (0: off), in which a forward records the hit lists
                                   it gathered; the next backward over the same ray set (same
                                   arrays and view, no rg_prepare / rg_set_views / rg_set_config

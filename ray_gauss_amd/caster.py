@@ -25,10 +25,10 @@ class TuneConfig:
     words per ray in which a forward records its hit lists for the backward over the same rays
     (0: the backward walks the tree again; 1536 words = 6 KB of device memory per ray)."""
 
-    hit_depth: int = 8
+    hit_depth: int = 16
     seg_slabs: int = 8
     packet: bool = True
-    fwd_hit_depth: int = 6
+    fwd_hit_depth: int = 12
     replay_words: int = 1536
 
 

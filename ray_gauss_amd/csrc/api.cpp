@@ -108,9 +108,9 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->hit_depth != 8 && c->hit_depth != 16 && c->hit_depth != 32)
         return fail(h, RG_ERR_ARG, "hit_depth must be 8, 16 or 32");
     if (c->seg_slabs < 1 || c->seg_slabs > 4096) return fail(h, RG_ERR_ARG, "seg_slabs in 1..4096");
-    if (c->fwd_hit_depth != 0 && c->fwd_hit_depth != 6 && c->fwd_hit_depth != 8 && c->fwd_hit_depth != 16 &&
-        c->fwd_hit_depth != 32)
-        return fail(h, RG_ERR_ARG, "fwd_hit_depth must be 0, 6, 8, 16 or 32");
+    if (c->fwd_hit_depth != 0 && c->fwd_hit_depth != 6 && c->fwd_hit_depth != 8 && c->fwd_hit_depth != 12 &&
+        c->fwd_hit_depth != 16 && c->fwd_hit_depth != 32)
+        return fail(h, RG_ERR_ARG, "fwd_hit_depth must be 0, 6, 8, 12, 16 or 32");
     if (c->jitter != 0 && c->jitter != 1) return fail(h, RG_ERR_ARG, "jitter must be 0 or 1");
     if (c->rnd_sample < 0 || c->rnd_sample > 2) return fail(h, RG_ERR_ARG, "rnd_sample must be 0, 1 or 2");
     if (c->replay_words < 0 || c->replay_words > (1 << 20) || (c->replay_words & 7))

@@ -56,9 +56,6 @@ struct IList {
     int g[CAP][kWave];
     float r1[CAP][kWave];
     float r2[CAP][kWave];
-    float tc[CAP][kWave];
-    float A[CAP][kWave];
-    float Cq[CAP][kWave];
 };
 
 // Bit e set for the list entries (e < cnt) whose interval meets the slab [tmin, tmax]: the
@@ -74,16 +71,6 @@ __device__ __forceinline__ unsigned int slab_mask(const IList<CAP>& L, int cnt, 
 }
 
 // Sampling quadratic of list entry e (as computed by intersect + refine_sampling at gather time).
-template <int CAP>
-__device__ __forceinline__ Isect entry_isect(const IList<CAP>& L, int e, int lane) {
-    Isect I;
-    I.r1 = L.r1[e][lane];
-    I.r2 = L.r2[e][lane];
-    I.tc = L.tc[e][lane];
-    I.A = L.A[e][lane];
-    I.Cq = L.Cq[e][lane];
-    return I;
-}
 
 struct Counters {
     unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0, rec_words = 0, rec_full = 0;
@@ -122,6 +109,19 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ geo, int s, 
     g1 = geo[4 * s + 1];
     g2 = geo[4 * s + 2];
     g3 = geo[4 * s + 3];
+}
+
+// Sampling terms of a listed hit, recomputed from the Gaussian's record (a list entry keeps only
+// the slot and the interval, 12 bytes, so the list holds twice the entries in the same LDS).
+// The same operations as when the list was gathered give the same values.
+__device__ __forceinline__ Isect entry_hit(const DevScene& sc, const Ray& ray, int slot, float& sigma) {
+    float4 g0, g1, g2, g3;
+    load_rec(sc.geo, slot, g0, g1, g2, g3);
+    Isect I;
+    intersect(g0, g1, g2, g3, ray, I);
+    refine_sampling(g0, g1, g2, g3, ray, I);
+    sigma = g0.w;
+    return I;
 }
 
 // Accumulate one hit's contribution to the slab's sample buffers (computeBufferForward,
@@ -368,13 +368,9 @@ __device__ __forceinline__ bool load_entry(const DevScene& sc, const Ray& ray, i
     load_rec(sc.geo, slot, g0, g1, g2, g3);
     Isect I;
     if (!intersect(g0, g1, g2, g3, ray, I)) return false;
-    refine_sampling(g0, g1, g2, g3, ray, I);
     L.g[e][lane] = slot;
     L.r1[e][lane] = I.r1;
     L.r2[e][lane] = I.r2;
-    L.tc[e][lane] = I.tc;
-    L.A[e][lane] = I.A;
-    L.Cq[e][lane] = I.Cq;
     return true;
 }
 
@@ -396,13 +392,9 @@ __device__ __forceinline__ int gather(const DevScene& sc, const DevParams& p, co
             cnt = CAP + 1;
             return false;
         }
-        refine_sampling(g0, g1, g2, g3, ray, I);
         L.g[cnt][lane] = slot;
         L.r1[cnt][lane] = I.r1;
         L.r2[cnt][lane] = I.r2;
-        L.tc[cnt][lane] = I.tc;
-        L.A[cnt][lane] = I.A;
-        L.Cq[cnt][lane] = I.Cq;
         ++cnt;
         return true;
     });
@@ -779,7 +771,9 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
                 const int e = __builtin_ctz(m);
                 m &= m - 1u;
                 const int slot = L.g[e][lane];
-                eval_hit(slot, entry_isect(L, e, lane), sc.geo[4 * slot].w);
+                float sigma;
+                const Isect I = entry_hit(sc, ray, slot, sigma);
+                eval_hit(slot, I, sigma);
             }
         }
         ctr.K += nh;
@@ -926,8 +920,9 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         const int e = __builtin_ctz(m);
                         m &= m - 1u;
                         const int slot = L.g[e][lane];
-                        accum_hit<SPB, RNG>(entry_isect(L, e, lane), sc.geo[4 * slot].w, rgbv[slot], tb, seed, p, sig,
-                                       cr, cg, cb);
+                        float sigma;
+                        const Isect I = entry_hit(sc, ray, slot, sigma);
+                        accum_hit<SPB, RNG>(I, sigma, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
                         ++nh;
                     }
                 }
@@ -1198,7 +1193,9 @@ __global__ void k_reset_counters(unsigned long long* c) {
 #define RG_DISPATCH_CAP6(SPBC, RNGC, RMC, CAPV, KERNEL, ...)                                                      \
     do {                                                                                                             \
         if (CAPV <= 6) hipLaunchKernelGGL((KERNEL<SPBC, 6, RNGC, RMC>), __VA_ARGS__);                                \
-        else RG_DISPATCH_CAP(SPBC, RNGC, RMC, CAPV, KERNEL, __VA_ARGS__);                                            \
+        else if (CAPV <= 12) hipLaunchKernelGGL((KERNEL<SPBC, 12, RNGC, RMC>), __VA_ARGS__);                         \
+        else if (CAPV <= 16) hipLaunchKernelGGL((KERNEL<SPBC, 16, RNGC, RMC>), __VA_ARGS__);                         \
+        else hipLaunchKernelGGL((KERNEL<SPBC, 32, RNGC, RMC>), __VA_ARGS__);                                         \
     } while (0)
 // Kernels are specialised on SPB (8 or 16 samples per slab), the list capacity, on whether the
 // random generator is used at all (jitter or rnd_sample), which costs registers, and on the
