@@ -412,6 +412,7 @@ struct SegState {
     float tb;       // start of the current slab (reference tbuffer)
     int k;          // index of that slab along the ray
     float seg_hi;   // end of the gathered segment
+    float seg_lo;   // start of the gathered segment
     int m;          // segment length in slabs for the next gather
     int cnt;        // entries in the lane's list
     bool valid;     // list holds the current segment
@@ -425,6 +426,7 @@ __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevP
     s.tb = ray.tenter;
     s.k = 0;
     s.seg_hi = ray.tenter;
+    s.seg_lo = ray.tenter;
     s.m = p.seg_slabs;
     s.cnt = 0;
     s.valid = false;
@@ -593,6 +595,7 @@ __device__ __forceinline__ bool next_slab(SegState& s, const DevScene& sc, const
         }
         if (!s.valid) {
             s.seg_hi = fminf(ray.texit, s.tb + (float)s.m * slab);
+            s.seg_lo = s.tb;
             const int cnt = gather<CAP>(sc, p, ray, s.tb, s.seg_hi, L, lane);
             ctr.gathers++;
             if (cnt > CAP) {
@@ -646,7 +649,10 @@ __device__ __forceinline__ bool needs_gather(const SegState& s, const Ray& ray, 
 // lists hold the same hit sets, so the result does not change.  Lanes replaying recorded lists
 // (own = false) neither gather nor ask others to.
 __device__ __forceinline__ void sync_gathers(SegState& s, const Ray& ray, const DevParams& p, bool own = true) {
-    if (__ballot(own && needs_gather(s, ray, p)) != 0ULL && own) s.valid = false;
+    const bool need = own && needs_gather(s, ray, p);
+    // a lane that has used up less than half of its segment keeps its list for a later round
+    const bool fresh = s.valid && !s.direct && s.cnt > 0 && 2.0f * (s.tb - s.seg_lo) < s.seg_hi - s.seg_lo;
+    if (__ballot(need) != 0ULL && own && (need || !fresh)) s.valid = false;
 }
 
 __device__ __forceinline__ bool advance_slab(SegState& s, const DevParams& p) {
