@@ -252,19 +252,23 @@ __device__ __forceinline__ void traverse_packet(const float4* __restrict__ nodes
                                                 float ta, float tb, F&& on_leaf) {
     const float ix = 1.0f / r.dx, iy = 1.0f / r.dy, iz = 1.0f / r.dz;
     bool live = active;
+    // the live lanes as a scalar mask, so that the ballot of a box test (a compare result) needs
+    // one scalar AND; a ballot of the loop-carried lane flag itself costs two vector instructions
+    unsigned long long livem = __builtin_amdgcn_ballot_w64(live);
     int idx = 0;
-    while (idx < total && __ballot(live) != 0ULL) {
+    while (idx < total && livem != 0ULL) {
         idx = __builtin_amdgcn_readfirstlane(idx);
         const float4 lo = ld_node(nodes, 2 * idx);
         const float4 hi = ld_node(nodes, 2 * idx + 1);
         const int miss = __float_as_int(lo.w);
         const int leaf = __float_as_int(hi.w);
         if (miss <= idx) return;  // malformed link: never loop (idx strictly increases)
-        const bool h = live && box_hit(lo, hi, r, ix, iy, iz, ta, tb);
-        if (__ballot(h) != 0ULL) {
+        const bool b = box_hit(lo, hi, r, ix, iy, iz, ta, tb);
+        if ((__builtin_amdgcn_ballot_w64(b) & livem) != 0ULL) {
             if (leaf >= 0) {
                 if (leaf >= ((total + 1) >> 1)) return;  // malformed leaf index: stop
-                if (h && !on_leaf(leaf)) live = false;
+                if (live && b && !on_leaf(leaf)) live = false;
+                livem = __builtin_amdgcn_ballot_w64(live);
                 idx = miss;
             } else {
                 idx = idx + 1;

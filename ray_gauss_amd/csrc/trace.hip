@@ -1036,7 +1036,7 @@ __device__ __forceinline__ int64_t block_item(int64_t b, int64_t nblocks) {
 }
 
 template <int SPB, int CAP, bool RNG, int RM>
-__global__ void __launch_bounds__(kWave) k_forward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB == 8 ? 4 : 3))) k_forward(DevScene sc, DevParams p, RayJob job, int64_t num_items,
                                                    float* __restrict__ out_rgb, float* __restrict__ out_depth,
                                                    float* __restrict__ out_opac, DevStats st) {
     __shared__ IList<CAP> L;
