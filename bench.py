@@ -153,7 +153,7 @@ def full_step(args):
     R = int(rv.shape[0])
     target = torch.rand((R, 3), generator=torch.Generator(device=dev).manual_seed(0), device=dev)
     tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
-                      replay_words=args.replay_words)
+                      replay_words=args.replay_words, tree_builder=args.tree_builder)
     trainer = BatchedTrainer(RayCaster(wl.params, tune), buf, cams, rv, rp, target, loss_kind=0,
                              use_graph=not args.no_graph)
     opts = [FusedAdam([raw[k]], lr=FULL_STEP_LR[k]) for k in raw]
@@ -217,6 +217,7 @@ def main():
     ap.add_argument("--replay-words", type=int, default=1536,
                     help="words per ray of the forward's hit-list records replayed by the backward (0: off)")
     ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
+    ap.add_argument("--tree-builder", type=int, default=1, help="0: LBVH (Morton order), 1: binned SAH")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
     ap.add_argument("--full-step", action="store_true",
@@ -272,7 +273,7 @@ def main():
     px_dev = [torch.from_numpy(p).to(dev) for p in pixels]
 
     tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
-                      replay_words=args.replay_words)
+                      replay_words=args.replay_words, tree_builder=args.tree_builder)
     rc = RayCaster(wl.params, tune)
 
     def prepare(rebuild=False):
@@ -338,6 +339,22 @@ def main():
         torch.cuda.synchronize()
         dt_pv = (time.perf_counter() - t0) / args.steps
 
+    # tree builds (outside the timed step, which refits the tree it was set up with): wall time
+    # of rg_prepare(rebuild) incl. preprocessing, second of two calls per builder
+    build_ms = {}
+    for name, builder in (("lbvh", 0), ("sah", 1)):
+        tb = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
+                        replay_words=0, tree_builder=builder)
+        rc_t = RayCaster(wl.params, tb)
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rc_t.prepare(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"], t["sg"] if L else None,
+                         t["sharpness"] if L else None, t["lobe_axis"] if L else None, rebuild=True)
+            torch.cuda.synchronize()
+            build_ms[name] = (time.perf_counter() - t0) * 1e3
+        del rc_t
+
     P, K = st["pairs"], st["slab_hits"]
     B_bwd = backward_bytes(R, P)
     achieved = B_bwd / (bk_ms * 1e-3) / 1e9
@@ -361,7 +378,8 @@ def main():
                    "sh_degree": sc.sh_degree, "sg_lobes": L, "dt": wl.params.dt, "t_eps": wl.params.t_eps,
                    "hit_depth": tune.hit_depth, "seg_slabs": tune.seg_slabs,
                    "packet_traversal": tune.packet, "fwd_hit_depth": tune.fwd_hit_depth,
-                   "replay_words": tune.replay_words},
+                   "replay_words": tune.replay_words, "tree_builder": ("lbvh", "sah")[tune.tree_builder]},
+        "tree_build_ms": build_ms,
         "per_view_rays_per_s": (R / dt_pv) if dt_pv else None,
         "batched_over_per_view": (dt_pv / dt_b) if dt_pv else None,
         "counters": {"K_slab_hits": K, "P_pairs": P, "fallback_slabs": st["fallback_slabs"],

@@ -60,6 +60,10 @@ typedef struct rg_config {
                                   arrays and view, no rg_prepare / rg_set_views / rg_set_config
                                   in between) reads them instead of walking the LBVH again, and
                                   rays whose lists did not fit walk the tree for the rest      */
+    int32_t tree_builder;      /* BVH build at rg_prepare(rebuild): 0 LBVH (Morton order, a few
+                                  ms at 1M), 1 binned SAH (tens of ms; fewer node visits per
+                                  ray segment; reads back one counter per tree level, so not
+                                  under stream capture)                                       */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

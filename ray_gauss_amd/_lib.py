@@ -21,7 +21,8 @@ class Config(C.Structure):
                 ("exact_sh_grad", C.c_int32), ("hit_depth", C.c_int32), ("seg_slabs", C.c_int32),
                 ("max_hits_per_slab", C.c_int32), ("packet_traversal", C.c_int32),
                 ("jitter", C.c_int32), ("rnd_sample", C.c_int32), ("iteration", C.c_uint32),
-                ("fwd_hit_depth", C.c_int32), ("replay_words", C.c_int32)]
+                ("fwd_hit_depth", C.c_int32), ("replay_words", C.c_int32),
+                ("tree_builder", C.c_int32)]
 
 
 class CameraS(C.Structure):

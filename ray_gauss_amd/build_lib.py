@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libraygauss_hip.so")
-SOURCES = ["preprocess.hip", "lbvh.hip", "trace.hip", "optim.hip", "loss.hip", "densify.hip", "api.cpp"]
+SOURCES = ["preprocess.hip", "lbvh.hip", "sah.hip", "trace.hip", "optim.hip", "loss.hip", "densify.hip", "api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -115,6 +115,7 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->rnd_sample < 0 || c->rnd_sample > 2) return fail(h, RG_ERR_ARG, "rnd_sample must be 0, 1 or 2");
     if (c->replay_words < 0 || c->replay_words > (1 << 20) || (c->replay_words & 7))
         return fail(h, RG_ERR_ARG, "replay_words must be a multiple of 8 in 0..2^20");
+    if (c->tree_builder != 0 && c->tree_builder != 1) return fail(h, RG_ERR_ARG, "tree_builder must be 0 or 1");
     return RG_OK;
 }
 
@@ -146,6 +147,12 @@ int ensure_gaussian_capacity(rg_handle* h, int64_t n) {
     if ((rc = dalloc(h, &b.k_right, cap))) return rc;
     if ((rc = dalloc(h, &b.k_first, cap))) return rc;
     if ((rc = dalloc(h, &b.k_last, cap))) return rc;
+    if ((rc = dalloc(h, &b.sah_prims[0], 2 * cap))) return rc;
+    if ((rc = dalloc(h, &b.sah_prims[1], 2 * cap))) return rc;
+    const int64_t task_f4 = (rg::sah_task_bytes((int)cap) + 15) / 16;
+    if ((rc = dalloc(h, &b.sah_tasks[0], task_f4))) return rc;
+    if ((rc = dalloc(h, &b.sah_tasks[1], task_f4))) return rc;
+    if ((rc = dalloc(h, &b.sah_ctr, 4))) return rc;
     h->cap = cap;
     h->built = false;
     return RG_OK;
@@ -276,7 +283,8 @@ int rg_destroy(rg_handle* h) {
                     h->bvh.vals[0], h->bvh.vals[1], h->bvh.hist, h->bvh.perm, h->bvh.iperm, h->bvh.nodes,
                     h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
                     h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb,
-                    h->train_rgb, h->rec, h->rec_len};
+                    h->train_rgb, h->rec, h->rec_len, h->bvh.sah_prims[0], h->bvh.sah_prims[1],
+                    h->bvh.sah_tasks[0], h->bvh.sah_tasks[1], h->bvh.sah_ctr};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete h;
@@ -317,7 +325,16 @@ int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* strea
         // records in original order -> Morton order -> tree
         HIPCHK(h, rg::launch_iota(n, h->bvh.perm, h->bvh.iperm, s));
         HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
-        HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
+        if (h->cfg.tree_builder == 1) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+                return fail(h, RG_ERR_ARG, "tree_builder 1 (SAH) cannot build under stream capture");
+            const hipError_t e = rg::launch_sah_build(n, h->leafbox, h->geo, h->box, h->bvh, s);
+            if (e == hipErrorUnknown) return fail(h, RG_ERR_HIP, "SAH build: inconsistent partition (internal error)");
+            HIPCHK(h, e);
+        } else {
+            HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
+        }
         HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
     }
     HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));

@@ -335,12 +335,19 @@ hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, Bvh
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s, n, leafbox, box, b.keys[0], b.vals[0]);
     const int cur = launch_radix_sort(n, 8, b.keys, b.vals, b.hist, s);
-    hipLaunchKernelGGL(k_set_perm, dim3(blocks(n, 256)), dim3(256), 0, s, n, b.vals[cur], b.perm, b.iperm);
     const int total = 2 * n - 1;
     hipLaunchKernelGGL(k_fill, dim3(blocks(total, 256)), dim3(256), 0, s, b.k_parent, total, -1);
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(blocks(n - 1, 256)), dim3(256), 0, s, n, b.keys[cur], b.k_left, b.k_right,
                            b.k_first, b.k_last, b.k_parent);
+    return launch_lbvh_finish(n, b.vals[cur], b, s);
+}
+
+// Slot order and preorder renumbering of a Karras-form topology whose leaves are numbered in
+// depth-first order (slot s = leaf s; sorted_vals[s] = original index).
+hipError_t launch_lbvh_finish(int n, const int* sorted_vals, BvhBuffers& b, hipStream_t s) {
+    const int total = 2 * n - 1;
+    hipLaunchKernelGGL(k_set_perm, dim3(blocks(n, 256)), dim3(256), 0, s, n, sorted_vals, b.perm, b.iperm);
     hipLaunchKernelGGL(k_preorder, dim3(blocks(total, 256)), dim3(256), 0, s, n, b.k_left, b.k_first, b.k_last,
                        b.k_parent, b.nodes, b.parent, b.right, b.leaf_node);
     return hipGetLastError();

@@ -103,8 +103,18 @@ struct BvhBuffers {
     int* k_right;
     int* k_first;
     int* k_last;
+    float4* sah_prims[2];   // binned-SAH build: (box, original index) records, ping-pong
+    float4* sah_tasks[2];   // binned-SAH build: node tasks of one level, ping-pong
+    int* sah_ctr;           // binned-SAH build: counters
 };
 hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, BvhBuffers& b, hipStream_t s);
+// Binned-SAH build (sah.hip): same outputs as launch_lbvh_build (geo: the preprocessed records in
+// original order, for the means of Gaussians without a box); synchronises the stream once per
+// tree level.  sah_task_bytes(n): size of each sah_tasks array.
+hipError_t launch_sah_build(int n, const float4* leafbox, const float4* geo, const float* box, BvhBuffers& b,
+                            hipStream_t s);
+int64_t sah_task_bytes(int n);
+hipError_t launch_lbvh_finish(int n, const int* sorted_vals, BvhBuffers& b, hipStream_t s);
 hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b, hipStream_t s);
 hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s);
 hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s);

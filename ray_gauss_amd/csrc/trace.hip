@@ -11,7 +11,8 @@
 // halving on overflow); a slab whose own hit set exceeds the list is processed by a per-slab
 // traversal that accumulates on the fly (no storage).  Forward and backward step through slabs
 // with the same SegState machine, so both make identical segment decisions and sum each slab in
-// the same order (the traversal's DFS order, which is increasing Morton slot).
+// the same order (the traversal's DFS order, which is increasing slot: slots are numbered in the
+// tree's depth-first leaf order by either builder).
 //
 // Sampling uses the closed form of the Gaussian along the ray: with the predicate's quantities
 // (t_c = b l, P = O' + b D^, D' = M' d, h = ln|sigma/thr|),
@@ -20,7 +21,7 @@
 // (slab, hit), four scalars (sum w phi, sum w phi tau, sum w phi tau^2, sum w B) and expands
 // them into the 14 parameter gradients of backward/gaussians_aabb.cu:700-763.
 //
-// Gradient scatter is wave-cooperative: every lane's list is sorted by Morton slot, so after each
+// Gradient scatter is wave-cooperative: every lane's list is sorted by slot, so after each
 // lane has prepared one slab, the 64 lanes (an 8x8 pixel tile) walk the union of their hit sets in
 // increasing slot order (a wave-wide min per step); the lanes holding the current Gaussian compute
 // their contributions together, and the wave sums them and issues the 14 atomic adds as one
@@ -376,7 +377,7 @@ __device__ __forceinline__ bool load_entry(const DevScene& sc, const Ray& ray, i
 
 // Gathers the intervals overlapping [ta, tb] of the Gaussians with slot > after into the lane's
 // list; returns the count, or CAP + 1 on overflow (the list then holds the first CAP).  Entries
-// come out in DFS order of the tree = increasing Morton slot.
+// come out in DFS order of the tree = increasing slot.
 template <int CAP>
 __device__ __forceinline__ int gather(const DevScene& sc, const DevParams& p, const Ray& ray, float ta, float tb,
                                       IList<CAP>& L, int lane, int after = -1) {

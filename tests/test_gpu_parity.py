@@ -109,7 +109,7 @@ def test_backward_parity_small(small):
 def test_tuning_invariance(gpu_device, depth, seg, packet):
     # the reference case uses the 6-entry forward list, the others the backward's depth
     """Hit-buffer depth, segment length and packet traversal change the schedule, not the image:
-    every schedule sums each slab's hits in the same (Morton slot) order, so the images are
+    every schedule sums each slab's hits in the same (slot) order, so the images are
     bit-identical."""
     wl = scenes.workload("small4k")
     cam = wl.cameras[0].launch_record()
@@ -144,10 +144,11 @@ def test_traversal_modes_agree(gpu_device):
         assert max_rel(out[True][1][k], v) <= 1e-5, k
 
 
-def test_blobs_crop_parity(gpu_device):
+@pytest.mark.parametrize("builder", [0, 1])
+def test_blobs_crop_parity(gpu_device, builder):
     wl = scenes.workload("blobs100k")
     cam = wl.cameras[0].launch_record()
-    rc, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device)
+    rc, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device, TuneConfig(tree_builder=builder))
     assert rc.check_bvh() == 0
     px = torch.from_numpy(scenes.crop_pixels(wl.cameras[0], 384, 384, 32, 32))
     rgb, _, _ = rc.forward(0, px)
