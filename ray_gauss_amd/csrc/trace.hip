@@ -125,25 +125,41 @@ __device__ __forceinline__ Isect entry_hit(const DevScene& sc, const Ray& ray, i
     return I;
 }
 
+// The Gaussian's weight along the ray, exp(-(A + Cq tau^2)), as one native exp2 with -log2(e)
+// folded into the per-hit terms; every pass (forward, backward buffers, gradients) evaluates it
+// the same way, so they agree bit for bit.  Weights that small enough to flush to zero are far
+// below the sample threshold.
+struct WExp {
+    float a, c;  // -A log2(e), -Cq log2(e)
+};
+
+__device__ __forceinline__ WExp wexp(const Isect& I) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    return WExp{-I.A * kLog2e, -I.Cq * kLog2e};
+}
+
+__device__ __forceinline__ float weight(const WExp& k, float tau) {
+    return __builtin_amdgcn_exp2f(fmaf(k.c, tau * tau, k.a));  // explicit fma: same rounding in every pass
+}
+
 // Accumulate one hit's contribution to the slab's sample buffers (computeBufferForward,
 // forward/gaussians_aabb.cu:437-498).
 template <int SPB, bool RNG>
 __device__ __forceinline__ void accum_hit(const Isect& I, float sigma, float4 col, float tb, unsigned int seed,
                                           const DevParams& p, float* sig, float* cr, float* cg, float* cb) {
     unsigned int st = seed;
+    const WExp k = wexp(I);
 #pragma unroll
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
         const float t = sample_t(tb, i, p.dt, RNG ? p.rnd_sample : 0, st);
-        const float tau = t - I.tc;
-        const float pw = fmaf(I.Cq, tau * tau, I.A);  // explicit: same rounding in every pass
-        const float sw = sigma * __expf(-pw);
-        if (sw > p.sigma_thr) {
-            sig[i] += sw;
-            cr[i] += sw * col.x;
-            cg[i] += sw * col.y;
-            cb[i] += sw * col.z;
-        }
+        const float sw = sigma * weight(k, t - I.tc);
+        // a sample at or below the threshold adds exact zeros (no branch per sample)
+        const float m = sw > p.sigma_thr ? sw : 0.0f;
+        sig[i] += m;
+        cr[i] += m * col.x;
+        cg[i] += m * col.y;
+        cb[i] += m * col.z;
     }
 }
 
@@ -216,13 +232,13 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
     const float F = dL[0] * col.x + dL[1] * col.y + dL[2] * col.z;
     float Ss = 0.0f, S1 = 0.0f, S2 = 0.0f, Sc = 0.0f;
     unsigned int st = seed;
+    const WExp k = wexp(I);
 #pragma unroll
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
         const float t = sample_t(tb, i, p.dt, RNG ? p.rnd_sample : 0, st);
         const float tau = t - I.tc;
-        const float pw = fmaf(I.Cq, tau * tau, I.A);  // explicit: same rounding in every pass
-        const float w = __expf(-pw);
+        const float w = weight(k, tau);
         if (sigma * w > p.sigma_thr) {
             const float phi = Aq[i] + Bq[i] * (F - Eq[i]);
             const float wp = w * phi;
