@@ -238,15 +238,14 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
         if (SPB > 8 && i >= p.spb) break;
         const float t = sample_t(tb, i, p.dt, RNG ? p.rnd_sample : 0, st);
         const float tau = t - I.tc;
-        const float w = weight(k, tau);
-        if (sigma * w > p.sigma_thr) {
-            const float phi = Aq[i] + Bq[i] * (F - Eq[i]);
-            const float wp = w * phi;
-            Ss += wp;
-            S1 += wp * tau;
-            S2 += wp * (tau * tau);
-            Sc += w * Bq[i];
-        }
+        const float w0 = weight(k, tau);
+        const float w = sigma * w0 > p.sigma_thr ? w0 : 0.0f;  // below the threshold: exact zeros
+        const float phi = Aq[i] + Bq[i] * (F - Eq[i]);
+        const float wp = w * phi;
+        Ss += wp;
+        S1 += wp * tau;
+        S2 += wp * (tau * tau);
+        Sc += w * Bq[i];
     }
     const float4 b0 = sc.brec[2 * slot], q = sc.brec[2 * slot + 1];
     const float d = b0.w;
