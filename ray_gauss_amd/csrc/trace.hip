@@ -173,12 +173,9 @@ __device__ __forceinline__ void composite(const float* sig, const float* cr, con
         if (SPB > 8 && i >= p.spb) break;
         const float s = sig[i];
         const float alpha = 1.0f - expf(-s * p.dt);
-        float r = cr[i], g = cg[i], b = cb[i];
-        if (s > 0.0f) {
-            r /= s;
-            g /= s;
-            b /= s;
-        }
+        // colour = sums / sigma: one reciprocal per sample (a sample without hits has zero sums)
+        const float inv = s > 0.0f ? 1.0f / s : 0.0f;
+        const float r = cr[i] * inv, g = cg[i] * inv, b = cb[i] * inv;
         const float ta = o.T * alpha;
         o.C[0] += ta * r;
         o.C[1] += ta * g;
@@ -201,14 +198,11 @@ __device__ __forceinline__ void slab_coeffs(float* sig, float* cr, float* cg, fl
         if (SPB > 8 && i >= p.spb) break;
         const float s = sig[i];
         const float alpha = 1.0f - expf(-s * p.dt);
-        float r = cr[i], g = cg[i], b = cb[i];
-        if (s > 0.0f) {
-            r /= s;
-            g /= s;
-            b /= s;
-        }
+        // the forward's colour = sums / sigma, evaluated as there (composite)
+        const float inv = s > 0.0f ? 1.0f / s : 0.0f;
+        const float r = cr[i] * inv, g = cg[i] * inv, b = cb[i] * inv;
         const float A = p.dt * (dL[0] * (T * r - Crem[0]) + dL[1] * (T * g - Crem[1]) + dL[2] * (T * b - Crem[2]));
-        const float Bq = s > 0.0f ? T * alpha / s : 0.0f;
+        const float Bq = T * alpha * inv;
         const float E = dL[0] * r + dL[1] * g + dL[2] * b;
         const float ta = T * alpha;
         Crem[0] -= ta * r;
