@@ -163,6 +163,14 @@ __device__ __forceinline__ void accum_hit(const Isect& I, float sigma, float4 co
     }
 }
 
+// Opacity of one sample, 1 - exp(-sigma dt) (forward/gaussians_aabb.cu:509), with the native
+// exp2 (1 ulp) and the native reciprocal for the colour below: the forward's compositing and the
+// backward's coefficients use the same two, so the backward retraces the forward's transmittance.
+__device__ __forceinline__ float sample_alpha(float s, float dt) {
+    constexpr float kNegLog2e = -1.4426950408889634f;
+    return 1.0f - __builtin_amdgcn_exp2f(s * (dt * kNegLog2e));
+}
+
 // colorBlending (forward/gaussians_aabb.cu:500-519) + depth (cuda_gui/gaussians_aabb.cu:415).
 template <int SPB, bool RNG>
 __device__ __forceinline__ void composite(const float* sig, const float* cr, const float* cg, const float* cb, float tb,
@@ -172,9 +180,9 @@ __device__ __forceinline__ void composite(const float* sig, const float* cr, con
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
         const float s = sig[i];
-        const float alpha = 1.0f - expf(-s * p.dt);
+        const float alpha = sample_alpha(s, p.dt);
         // colour = sums / sigma: one reciprocal per sample (a sample without hits has zero sums)
-        const float inv = s > 0.0f ? 1.0f / s : 0.0f;
+        const float inv = s > 0.0f ? __builtin_amdgcn_rcpf(s) : 0.0f;
         const float r = cr[i] * inv, g = cg[i] * inv, b = cb[i] * inv;
         const float ta = o.T * alpha;
         o.C[0] += ta * r;
@@ -197,9 +205,9 @@ __device__ __forceinline__ void slab_coeffs(float* sig, float* cr, float* cg, fl
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
         const float s = sig[i];
-        const float alpha = 1.0f - expf(-s * p.dt);
+        const float alpha = sample_alpha(s, p.dt);
         // the forward's colour = sums / sigma, evaluated as there (composite)
-        const float inv = s > 0.0f ? 1.0f / s : 0.0f;
+        const float inv = s > 0.0f ? __builtin_amdgcn_rcpf(s) : 0.0f;
         const float r = cr[i] * inv, g = cg[i] * inv, b = cb[i] * inv;
         const float A = p.dt * (dL[0] * (T * r - Crem[0]) + dL[1] * (T * g - Crem[1]) + dL[2] * (T * b - Crem[2]));
         const float Bq = T * alpha * inv;
@@ -260,9 +268,11 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
     float dcol[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = d * (m[0][c] * Gy[0] + m[1][c] * Gy[1] + m[2][c] * Gy[2]);
+    // reciprocal scales: native reciprocal (1 ulp), well inside the gradients' tolerance
+    const float is[3] = {__builtin_amdgcn_rcpf(s[0]), __builtin_amdgcn_rcpf(s[1]), __builtin_amdgcn_rcpf(s[2])};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float f = -d / s[k];
+        const float f = -d * is[k];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             dcol[k][c] = f * (s[0] * s[0] * Y[0][k] * m[0][c] + s[1] * s[1] * Y[1][k] * m[1][c] +
@@ -274,9 +284,9 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
     const float* c1 = dcol[1];
     const float* c2 = dcol[2];
     v[3] = Ss;
-    v[4] = Y[0][0] / s[0];
-    v[5] = Y[1][1] / s[1];
-    v[6] = Y[2][2] / s[2];
+    v[4] = Y[0][0] * is[0];
+    v[5] = Y[1][1] * is[1];
+    v[6] = Y[2][2] * is[2];
     v[7] = 2.0f * (qk * c0[1] - qj * c0[2] - qk * c1[0] + qi * c1[2] + qj * c2[0] - qi * c2[1]);
     v[8] = 2.0f * (qj * c0[1] + qk * c0[2] + qj * c1[0] + r * c1[2] + qk * c2[0] - r * c2[1]) -
            4.0f * qi * (c1[1] + c2[2]);
