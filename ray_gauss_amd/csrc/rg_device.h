@@ -117,13 +117,11 @@ struct Isect {
     float l;              // 1 / |D'|
 };
 
-// __intersection__gaussian (reference cuda_train/forward/gaussians_aabb.cu:324-402), exact FP32
-// for the decision part.  Returns true when disc > 1e-7; the hit test on [tmin, tmax] is then
-// fmaxf(tmin, r1) <= fminf(tmax, r2).  The operation order is shared with the host FP32 checker.
-__device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
-                                          const Ray& r, Isect& I) {
+// Sampling terms of the bounding-ellipsoid test (everything but the roots), exact FP32 with the
+// operation order of the host FP32 checker.  Returns disc = 1 - |P|^2.
+__device__ __forceinline__ float isect_terms(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
+                                             const Ray& r, Isect& I, float& b, float& c) {
 #pragma clang fp contract(off)
-    if (!(g1.w > 0.0f)) return false;  // h <= 0 or NaN: no bounding ellipsoid (|sigma| <= thr)
     const float Ox = r.ox - g0.x, Oy = r.oy - g0.y, Oz = r.oz - g0.z;
     const float Op0 = (g1.x * Ox + g1.y * Oy) + g1.z * Oz;
     const float Op1 = (g2.x * Ox + g2.y * Oy) + g2.z * Oz;
@@ -134,16 +132,10 @@ __device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, cons
     const float ddp = (Dp0 * Dp0 + Dp1 * Dp1) + Dp2 * Dp2;
     const float l = 1.0f / __builtin_sqrtf(ddp);
     const float Dh0 = Dp0 * l, Dh1 = Dp1 * l, Dh2 = Dp2 * l;
-    const float b = -((Op0 * Dh0 + Op1 * Dh1) + Op2 * Dh2);
-    const float c = ((Op0 * Op0 + Op1 * Op1) + Op2 * Op2) - 1.0f;
+    b = -((Op0 * Dh0 + Op1 * Dh1) + Op2 * Dh2);
+    c = ((Op0 * Op0 + Op1 * Op1) + Op2 * Op2) - 1.0f;
     const float P0 = Op0 + b * Dh0, P1 = Op1 + b * Dh1, P2 = Op2 + b * Dh2;
     const float pp = (P0 * P0 + P1 * P1) + P2 * P2;
-    const float disc = 1.0f - pp;
-    if (!(disc > 1e-7f)) return false;
-    const float sd = __builtin_sqrtf(disc);
-    const float q = (b > 0.0f) ? b + sd : b - sd;
-    I.r1 = (c / q) * l;
-    I.r2 = q * l;
     I.tc = b * l;
     I.A = g1.w * pp;
     I.Cq = g1.w * ddp;
@@ -154,6 +146,23 @@ __device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, cons
     I.Dy = Dp1;
     I.Dz = Dp2;
     I.l = l;
+    return 1.0f - pp;
+}
+
+// __intersection__gaussian (reference cuda_train/forward/gaussians_aabb.cu:324-402), exact FP32
+// for the decision part.  Returns true when disc > 1e-7; the hit test on [tmin, tmax] is then
+// fmaxf(tmin, r1) <= fminf(tmax, r2).  The operation order is shared with the host FP32 checker.
+__device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
+                                          const Ray& r, Isect& I) {
+#pragma clang fp contract(off)
+    if (!(g1.w > 0.0f)) return false;  // h <= 0 or NaN: no bounding ellipsoid (|sigma| <= thr)
+    float b, c;
+    const float disc = isect_terms(g0, g1, g2, g3, r, I, b, c);
+    if (!(disc > 1e-7f)) return false;
+    const float sd = __builtin_sqrtf(disc);
+    const float q = (b > 0.0f) ? b + sd : b - sd;
+    I.r1 = (c / q) * I.l;
+    I.r2 = q * I.l;
     return true;
 }
 

@@ -114,12 +114,16 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ geo, int s, 
 
 // Sampling terms of a listed hit, recomputed from the Gaussian's record (a list entry keeps only
 // the slot and the interval, 12 bytes, so the list holds twice the entries in the same LDS).
-// The same operations as when the list was gathered give the same values.
-__device__ __forceinline__ Isect entry_hit(const DevScene& sc, const Ray& ray, int slot, float& sigma) {
+// The same operations as the predicate's give the same values; the roots are skipped.
+__device__ __forceinline__ Isect entry_hit(const DevScene& sc, const Ray& ray, int slot, float r1, float r2,
+                                           float& sigma) {
     float4 g0, g1, g2, g3;
     load_rec(sc.geo, slot, g0, g1, g2, g3);
     Isect I;
-    intersect(g0, g1, g2, g3, ray, I);
+    float b, c;
+    isect_terms(g0, g1, g2, g3, ray, I, b, c);  // the interval is the entry's: no roots needed
+    I.r1 = r1;
+    I.r2 = r2;
     refine_sampling(g0, g1, g2, g3, ray, I);
     sigma = g0.w;
     return I;
@@ -798,7 +802,7 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
                 m &= m - 1u;
                 const int slot = L.g[e][lane];
                 float sigma;
-                const Isect I = entry_hit(sc, ray, slot, sigma);
+                const Isect I = entry_hit(sc, ray, slot, L.r1[e][lane], L.r2[e][lane], sigma);
                 eval_hit(slot, I, sigma);
             }
         }
@@ -947,7 +951,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         m &= m - 1u;
                         const int slot = L.g[e][lane];
                         float sigma;
-                        const Isect I = entry_hit(sc, ray, slot, sigma);
+                        const Isect I = entry_hit(sc, ray, slot, L.r1[e][lane], L.r2[e][lane], sigma);
                         accum_hit<SPB, RNG>(I, sigma, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
                         ++nh;
                     }
