@@ -85,8 +85,8 @@ def step_bytes(R, P, n, K, L, V):
 
 
 def cpu_baseline(wl, threads=None):
-    """The CPU FP64 oracle (OpenMP over rays) on a bounded sample: a 192x192 crop at the centre
-    of view 0 (about 10-20 s of CPU work on 16 cores)."""
+    """The CPU FP64 oracle (OpenMP over rays) on a bounded sample: a 384x384 crop at the centre
+    of view 0 (about 10-15 s of CPU work on 16 cores)."""
     import numpy as np
 
     from oracle import oracle as O
@@ -95,7 +95,7 @@ def cpu_baseline(wl, threads=None):
     cam_obj = wl.cameras[0]
     cam = cam_obj.launch_record()
     w, h = cam["width"], cam["height"]
-    c = 192
+    c = 384
     px = scenes.crop_pixels(cam_obj, w // 2 - c // 2, h // 2 - c // 2, c, c)
     n_threads = O.nthreads(threads)
     rgb = O.sh_forward(wl.scene, cam["eye"], n_threads)
