@@ -1016,9 +1016,16 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 load_rec(sc.geo, slot, g0, g1, g2, g3);
                 Isect I;
                 // a recorded direct hit reproduces the forward's test (a record that does not is
-                // skipped rather than scattered)
-                const bool ok = intersect(g0, g1, g2, g3, ray, I) &&
-                                (!cur_dlist || overlaps(cur_tmin, cur_tmax, I.r1, I.r2));
+                // skipped rather than scattered); a list entry passed it when it was gathered, so
+                // only its sampling terms are computed (the same values, without the roots)
+                bool ok;
+                if (cur_dlist) {
+                    ok = intersect(g0, g1, g2, g3, ray, I) && overlaps(cur_tmin, cur_tmax, I.r1, I.r2);
+                } else {
+                    float b, c;
+                    isect_terms(g0, g1, g2, g3, ray, I, b, c);
+                    ok = true;
+                }
                 if (ok) {
                     refine_sampling(g0, g1, g2, g3, ray, I);
                     grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc,
