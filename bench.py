@@ -212,7 +212,7 @@ def main():
     ap.add_argument("--no-per-view", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="run the batched step eagerly instead of as a HIP graph")
     ap.add_argument("--hit-depth", type=int, default=16)
-    ap.add_argument("--seg-slabs", type=int, default=8)
+    ap.add_argument("--seg-slabs", type=int, default=32)
     ap.add_argument("--fwd-hit-depth", type=int, default=12, help="list depth of the forward kernel (6, 12, 16, 32)")
     ap.add_argument("--replay-words", type=int, default=1536,
                     help="words per ray of the forward's hit-list records replayed by the backward (0: off)")

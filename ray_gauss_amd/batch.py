@@ -115,7 +115,7 @@ class BatchedTrainer:
         self.rc.set_views(self.cams)
 
 
-def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32), segs=(4, 8, 16),
+def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32), segs=(8, 32, 64),
              packets=(True, False), reps: int = 2, key: str | None = None, save: bool = True):
     """Times the train step for every (hit_depth, seg_slabs, packet) and keeps the fastest.
     Returns (best TuneConfig, table of (config, ms))."""

@@ -30,7 +30,7 @@ class TuneConfig:
     refit every step."""
 
     hit_depth: int = 16
-    seg_slabs: int = 8
+    seg_slabs: int = 32
     packet: bool = True
     fwd_hit_depth: int = 12
     replay_words: int = 1536
