@@ -290,7 +290,7 @@ def main():
         for v in range(len(cams)):
             rgb, _, _ = rc.forward(v, px_dev[v])
             dL = 2.0 * (rgb - targets_pv[v])
-            rc.backward(v, px_dev[v], rgb, dL)
+            rc.backward(v, px_dev[v], rgb, dL, rc.last_token)
         rc.finish_grads()
 
     # ---- batched: refit + colour + forward + loss/backward + SH/SG backward, one HIP graph ----
@@ -319,7 +319,7 @@ def main():
         dL_b = 2.0 * (rgb_b - target)
         torch.cuda.synchronize()
         ev[0].record()
-        rc_b.backward_batch(ray_view, ray_pixel, rgb_b, dL_b)
+        rc_b.backward_batch(ray_view, ray_pixel, rgb_b, dL_b, rc_b.last_token)
         ev[1].record()
         torch.cuda.synchronize()
         b_ms.append(ev[0].elapsed_time(ev[1]))

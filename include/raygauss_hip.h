@@ -56,10 +56,12 @@ typedef struct rg_config {
                                   (0: hit_depth); a smaller list lets more wavefronts share a CU's
                                   LDS (12 bytes per entry and lane)                             */
     int32_t replay_words;      /* words per ray, a multiple of 8 (0: off), in which a forward records the hit lists
-                                  it gathered; the next backward over the same ray set (same
-                                  arrays and view, no rg_prepare / rg_set_views / rg_set_config
-                                  in between) reads them instead of walking the LBVH again, and
-                                  rays whose lists did not fit walk the tree for the rest      */
+                                  it gathered and returns a token for them; a backward given
+                                  that token (no rg_prepare / rg_set_views / rg_set_config /
+                                  other recording forward in between) reads them instead of
+                                  walking the LBVH again, for each ray whose (view, pixel) is
+                                  the one recorded in its output row; rays whose lists did not
+                                  fit walk the tree for the rest                               */
     int32_t tree_builder;      /* BVH build at rg_prepare(rebuild): 0 LBVH (Morton order, a few
                                   ms at 1M), 1 binned SAH (tens of ms; fewer node visits per
                                   ray segment; reads back one counter per tree level, so not
@@ -127,6 +129,10 @@ typedef struct rg_stats {
     int64_t gathers;          /* interval-list gathers (BVH traversals)                       */
     int64_t replay_words;     /* words of hit-list records written by the forward (rg_train_batch) */
     int64_t replay_truncated; /* rays whose records did not fit replay_words                    */
+    int64_t dropped;          /* gradient records or replayed hits skipped by a bounds check:
+                                 an internal error, never expected (tests assert zero)         */
+    int64_t bad_rays;         /* batch / pixel-list rays whose view or pixel index is out of
+                                 range: skipped (forward output NaN, no gradient)              */
 } rg_stats;
 
 /* Creates a handle.  Replaces create_context/module/pipeline (optix_utils.py:77-110): the
@@ -142,7 +148,8 @@ const char* rg_last_error(const rg_handle* h);
  * the LBVH (Morton codes, radix sort, radix tree); otherwise the existing tree is refit. */
 int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* stream);
 
-/* Uploads the views and computes each view's per-Gaussian colour table.  Replaces
+/* Uploads the views and computes each view's per-Gaussian colour table (num_views * n must stay
+ * below 2^31 - 1: RG_ERR_ARG otherwise).  Replaces
  * compute_cupy_rgb / forward_sh (render_optix.py:19-32, utils/sh_utils.cu:25-119) and the
  * camera part of the launch record (optix_utils.py:166-181). */
 int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* stream);
@@ -153,24 +160,25 @@ int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* s
 int rg_set_view_colors(rg_handle* h, int32_t view, const float* rgb, void* stream);
 
 /* Forward render of num_rays rays of one view (pixels == NULL: ray r is pixel r).  out_rgb
- * (num_rays,3) is required; out_depth / out_opacity (num_rays) are optional.  Replaces
- * launch_pipeline_forward (optix_utils.py:128-209) and the forward raygen
- * (cuda_train/forward/gaussians_aabb.cu:521-612). */
+ * (num_rays,3) is required; out_depth / out_opacity (num_rays) are optional.  out_token
+ * (optional, host) receives the token of the hit-list records this forward wrote (0: none; see
+ * rg_config.replay_words).  Replaces launch_pipeline_forward (optix_utils.py:128-209) and the
+ * forward raygen (cuda_train/forward/gaussians_aabb.cu:521-612). */
 int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, float* out_rgb,
-               float* out_depth, float* out_opacity, void* stream);
+               float* out_depth, float* out_opacity, uint64_t* out_token, void* stream);
 
 /* Backward of rg_forward for the same rays: accumulates parameter gradients into the handle.
- * ray_colors are rg_forward's outputs, dL_dC the incoming gradient (num_rays,3).  Replaces
- * launch_pipeline_backward (optix_utils.py:213-299) and the backward raygen
- * (cuda_train/backward/gaussians_aabb.cu:505-775). */
+ * ray_colors are rg_forward's outputs, dL_dC the incoming gradient (num_rays,3); replay_token
+ * is the forward's out_token (0: walk the LBVH again).  Replaces launch_pipeline_backward
+ * (optix_utils.py:213-299) and the backward raygen (cuda_train/backward/gaussians_aabb.cu:505-775). */
 int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, const float* ray_colors,
-                const float* dL_dC, void* stream);
+                const float* dL_dC, uint64_t replay_token, void* stream);
 
 /* Multi-view versions: ray r of the batch is pixel ray_pixel[r] of view ray_view[r]. */
 int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float* out_depth, float* out_opacity,
-                     void* stream);
+                     uint64_t* out_token, void* stream);
 int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colors, const float* dL_dC,
-                      void* stream);
+                      uint64_t replay_token, void* stream);
 
 /* Train step over a batch: forward, per-ray loss against target (loss_kind 0: L2 =
  * sum (C-target)^2, 1: L1 = sum |C-target|), backward -- two launches, the loss gradient formed
@@ -197,6 +205,12 @@ int rg_scene_box(rg_handle* h, float* out6_device, void* stream);
 
 /* Counters of the last forward/backward/train call (synchronises the stream). */
 int rg_get_stats(rg_handle* h, rg_stats* out, void* stream);
+
+/* Allocation epoch of the handle's device workspace: it changes whenever a call (re)allocates a
+ * buffer (a larger n, more views or rays, a first recording forward).  A HIP graph captured
+ * over this handle's calls bakes in those addresses and stays valid only while the epoch is
+ * unchanged.  No reference counterpart (the reference captures no graphs). */
+int64_t rg_alloc_epoch(const rg_handle* h);
 
 /* Debug/verification: the FP32 predicate parameters of the bounding ellipsoids in original
  * order -- mu (n,3), Mp (n,9) rows of S'^-1 R^T, valid (n) -- exactly as the kernels use them. */

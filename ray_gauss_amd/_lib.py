@@ -47,7 +47,7 @@ class RayBatch(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("slab_hits", C.c_int64), ("pairs", C.c_int64), ("overflow_slabs", C.c_int64),
                 ("fallback_slabs", C.c_int64), ("gathers", C.c_int64), ("replay_words", C.c_int64),
-                ("replay_truncated", C.c_int64)]
+                ("replay_truncated", C.c_int64), ("dropped", C.c_int64), ("bad_rays", C.c_int64)]
 
 
 class AdamTensor(C.Structure):
@@ -86,16 +86,20 @@ EXPORTS = {
     "rg_prepare": (C.c_int, [C.c_void_p, C.POINTER(Gaussians), C.c_int32, C.c_void_p]),
     "rg_set_views": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CameraS), C.c_void_p]),
     "rg_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                             C.c_void_p]),
-    "rg_backward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "rg_forward_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "rg_backward_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_void_p, C.c_void_p]),
+                             C.POINTER(C.c_uint64), C.c_void_p]),
+    "rg_backward": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                              C.c_void_p]),
+    "rg_forward_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_uint64), C.c_void_p]),
+    "rg_backward_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_void_p, C.c_uint64,
+                                    C.c_void_p]),
     "rg_train_batch": (C.c_int, [C.c_void_p, C.POINTER(RayBatch), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     "rg_zero_grads": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rg_finish_grads": (C.c_int, [C.c_void_p, C.POINTER(Grads), C.c_void_p]),
     "rg_scene_box": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rg_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats), C.c_void_p]),
+    "rg_alloc_epoch": (C.c_int64, [C.c_void_p]),
     "rg_export_predicate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rg_debug_hits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

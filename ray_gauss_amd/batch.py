@@ -5,7 +5,9 @@ autotuner.
 The graph holds only kernel launches and device-to-device copies (no host transfers, no
 memsets): camera upload happens once, outside the graph, in ``set_views``.  Parameters must be
 updated in place between replays (as an optimiser does); the gradients are returned in the
-graph's static output tensors.
+graph's static output tensors.  A graph bakes in device addresses: the trainer captures again
+whenever the Gaussian count, a parameter or ray tensor, or the handle's own workspace
+(``RayCaster.alloc_epoch``) has changed since the capture.
 """
 
 from __future__ import annotations
@@ -74,6 +76,7 @@ class BatchedTrainer:
         self.use_graph = use_graph
         self.loss = torch.zeros(1, dtype=torch.float32, device=ray_view.device)
         self.graph = None
+        self.graph_key = None
         self.grads = None
         L = params["sg"].shape[2] if params.get("sg") is not None else 0
         self.L = L
@@ -90,10 +93,18 @@ class BatchedTrainer:
         self.rc.train_batch(self.rv, self.rp, self.target, self.loss_kind, None, self.loss)
         return self.rc.finish_grads()
 
+    def _key(self):
+        """Everything a captured graph depends on: sizes, tensor addresses, workspace epoch."""
+        ptrs = tuple((k, None if v is None else (v.data_ptr(), tuple(v.shape))) for k, v in sorted(self.p.items()))
+        return (self.rc.n, ptrs, self.rv.data_ptr(), self.rp.data_ptr(), int(self.rv.shape[0]),
+                self.target.data_ptr(), self.rc.alloc_epoch())
+
     def step(self):
         if not self.use_graph:
             self.grads = self._pass()
             return self.grads
+        if self.graph is not None and self.graph_key != self._key():
+            self.graph = None  # stale addresses: capture again
         if self.graph is None:
             # warm up on a side stream, then capture
             s = torch.cuda.Stream()
@@ -105,14 +116,21 @@ class BatchedTrainer:
             self.graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph):
                 self.grads = self._pass()
+            self.graph_key = self._key()
         self.graph.replay()
         return self.grads
 
-    def rebuild(self):
-        """Rebuild the LBVH (new Morton order) outside the graph; the graph stays valid because
-        the handle's buffers keep their addresses when the Gaussian count is unchanged."""
+    def rebuild(self, params: dict | None = None):
+        """Rebuild the tree outside the graph, optionally over new parameter tensors (after
+        densification).  With the same Gaussian count and tensors the handle's buffers keep their
+        addresses and the graph stays valid; otherwise the next step captures again."""
+        if params is not None:
+            self.p = params
+            self.L = params["sg"].shape[2] if params.get("sg") is not None else 0
         self._prepare(rebuild=True)
         self.rc.set_views(self.cams)
+        if self.graph is not None and self.graph_key != self._key():
+            self.graph = None
 
 
 def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32), segs=(8, 32, 64),

@@ -90,6 +90,7 @@ class RayCaster:
         if rc != 0:
             raise _lib.RayGaussError(f"rg_create failed ({rc})")
         self._h = h
+        self.last_token = 0  # hit-list record token of the last forward (0: none recorded)
         self._inputs = None
         self._cams = []
         self.generation = 0
@@ -196,10 +197,15 @@ class RayCaster:
         d = torch.empty(nr, dtype=torch.float32, device=dev) if depth else None
         o = torch.empty(nr, dtype=torch.float32, device=dev) if opacity else None
         px = None if pixels is None else pixels.to(device=dev, dtype=torch.int32).contiguous()
-        _lib.check(self._L.rg_forward(self._h, view, nr, _ptr(px), _ptr(out), _ptr(d), _ptr(o), _stream()), self._h)
+        tok = C.c_uint64(0)
+        _lib.check(self._L.rg_forward(self._h, view, nr, _ptr(px), _ptr(out), _ptr(d), _ptr(o), C.byref(tok),
+                                      _stream()), self._h)
+        self.last_token = int(tok.value)
         return out, d, o
 
-    def backward(self, view: int, pixels, ray_colors, dLdC):
+    def backward(self, view: int, pixels, ray_colors, dLdC, replay_token: int = 0):
+        """Backward of forward(view, pixels); replay_token = the forward's ``last_token`` lets the
+        backward read that forward's recorded hit lists instead of walking the tree (0: walk)."""
         rec = self._cams[view]
         nr = rec["width"] * rec["height"] if pixels is None else int(pixels.shape[0])
         dev = self._inputs["positions"].device
@@ -208,7 +214,8 @@ class RayCaster:
         dl = dLdC.detach().contiguous().float()
         if rc.shape != (nr, 3) or dl.shape != (nr, 3):
             raise ValueError("ray_colors and dL/dC must be (num_rays, 3)")
-        _lib.check(self._L.rg_backward(self._h, view, nr, _ptr(px), _ptr(rc), _ptr(dl), _stream()), self._h)
+        _lib.check(self._L.rg_backward(self._h, view, nr, _ptr(px), _ptr(rc), _ptr(dl), int(replay_token),
+                                       _stream()), self._h)
 
     # ---------------------------------------------------------------- batches -----------------
     @staticmethod
@@ -226,13 +233,16 @@ class RayCaster:
         d = torch.empty(nr, dtype=torch.float32, device=dev) if depth else None
         o = torch.empty(nr, dtype=torch.float32, device=dev) if opacity else None
         b = self._batch(ray_view, ray_pixel)
-        _lib.check(self._L.rg_forward_batch(self._h, C.byref(b), _ptr(out), _ptr(d), _ptr(o), _stream()), self._h)
+        tok = C.c_uint64(0)
+        _lib.check(self._L.rg_forward_batch(self._h, C.byref(b), _ptr(out), _ptr(d), _ptr(o), C.byref(tok),
+                                            _stream()), self._h)
+        self.last_token = int(tok.value)
         return out, d, o
 
-    def backward_batch(self, ray_view, ray_pixel, ray_colors, dLdC):
+    def backward_batch(self, ray_view, ray_pixel, ray_colors, dLdC, replay_token: int = 0):
         b = self._batch(ray_view, ray_pixel)
         _lib.check(self._L.rg_backward_batch(self._h, C.byref(b), _ptr(ray_colors.contiguous()),
-                                             _ptr(dLdC.contiguous()), _stream()), self._h)
+                                             _ptr(dLdC.contiguous()), int(replay_token), _stream()), self._h)
 
     def train_batch(self, ray_view, ray_pixel, target, loss_kind=0, out_rgb=None, out_loss=None):
         b = self._batch(ray_view, ray_pixel)
@@ -293,6 +303,10 @@ class RayCaster:
         _lib.check(self._L.rg_debug_hits(self._h, view, nr, _ptr(px), max_slabs, max_pairs, _ptr(pairs), _ptr(counts),
                                          _ptr(ro), _ptr(rd), _ptr(te), _ptr(tx), _stream()), self._h)
         return pairs, counts, ro, rd, te, tx
+
+    def alloc_epoch(self) -> int:
+        """Changes whenever the handle (re)allocates device memory (graphs over it are then stale)."""
+        return int(self._L.rg_alloc_epoch(self._h))
 
     def check_bvh(self) -> int:
         e = C.c_int64()

@@ -2,6 +2,7 @@
 // workspace and sequences the kernels; never throws across the ABI.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -39,21 +40,23 @@ struct rg_handle {
     // colour buffer of rg_train_batch when the caller passes no out_rgb
     float* train_rgb = nullptr;
     int64_t train_rgb_cap = 0;
-    // hit-list records of the last forward (rays x replay_words) and the ray set they belong to
+    // hit-list records of the last forward (rays x replay_words), the (view, pixel) each row
+    // belongs to, and the token the forward returned for them
     unsigned int* rec = nullptr;
     int* rec_len = nullptr;
+    int2* rec_id = nullptr;
     int64_t rec_cap_words = 0;
     int64_t rec_cap_rays = 0;
+    uint64_t tokens_issued = 0;
     struct {
         bool valid = false;
+        uint64_t token = 0;
         int gen = 0;
-        const void* ray_view = nullptr;
-        const void* ray_pixel = nullptr;
-        const void* pixels = nullptr;
         int64_t num_rays = 0;
-        int view = 0;
         int words = 0;
     } rec_set;
+    // bumped by every device (re)allocation of the handle's workspace
+    int64_t alloc_epoch = 0;
 };
 
 namespace {
@@ -91,6 +94,7 @@ int dalloc(rg_handle* h, T** p, int64_t count) {
         *p = nullptr;
     }
     if (count <= 0) count = 1;
+    if (h) h->alloc_epoch++;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)count);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -204,6 +208,7 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.iteration = h->cfg.iteration;
     p.rec = nullptr;
     p.rec_len = nullptr;
+    p.rec_id = nullptr;
     p.rec_words = 0;
     p.rec_mode = 0;
     return p;
@@ -283,7 +288,7 @@ int rg_destroy(rg_handle* h) {
                     h->bvh.vals[0], h->bvh.vals[1], h->bvh.hist, h->bvh.perm, h->bvh.iperm, h->bvh.nodes,
                     h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
                     h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb,
-                    h->train_rgb, h->rec, h->rec_len, h->bvh.sah_prims[0], h->bvh.sah_prims[1],
+                    h->train_rgb, h->rec, h->rec_len, h->rec_id, h->bvh.sah_prims[0], h->bvh.sah_prims[1],
                     h->bvh.sah_tasks[0], h->bvh.sah_tasks[1], h->bvh.sah_ctr};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -315,35 +320,41 @@ int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* strea
         return fail(h, RG_ERR_ARG, "null SG array with num_sg > 0");
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc;
-    if ((rc = ensure_gaussian_capacity(h, g->n))) return rc;
     const bool topo = rebuild || !h->built || g->n != h->n;
+    if (topo && h->cfg.tree_builder == 1) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return fail(h, RG_ERR_ARG, "tree_builder 1 (SAH) cannot build under stream capture");
+    }
+    // from here on the handle's state changes: it is 'built' again only when everything succeeded
+    h->built = false;
+    h->rec_set.valid = false;
+    if ((rc = ensure_gaussian_capacity(h, g->n))) return rc;
     h->n = g->n;
     h->g = *g;
     h->generation++;
     const int n = (int)g->n;
     if (topo) {
+        h->num_views = 0;  // colour tables are per slot: views must be set again
         // records in original order -> Morton order -> tree
         HIPCHK(h, rg::launch_iota(n, h->bvh.perm, h->bvh.iperm, s));
         HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
         if (h->cfg.tree_builder == 1) {
-            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-            if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
-                return fail(h, RG_ERR_ARG, "tree_builder 1 (SAH) cannot build under stream capture");
             const hipError_t e = rg::launch_sah_build(n, h->leafbox, h->geo, h->box, h->bvh, s);
             if (e == hipErrorUnknown) return fail(h, RG_ERR_HIP, "SAH build: inconsistent partition (internal error)");
             HIPCHK(h, e);
         } else {
             HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
         }
+        // gradient accumulators are per slot: clear what an unfinished backward left in them
         HIPCHK(h, rg::launch_zero(h->acc, (int64_t)rg::kAccStride * std::max<int64_t>(h->cap, 1), s));
+        if (h->drgb) HIPCHK(h, rg::launch_zero(reinterpret_cast<float*>(h->drgb), 4 * h->rgb_cap, s));
     }
     HIPCHK(h, rg::launch_preprocess(*g, h->bvh.iperm, h->cfg.sigma_thr, h->geo, h->brec, h->leafbox, h->box, s));
     HIPCHK(h, rg::launch_lbvh_refit(n, h->leafbox, h->bvh, s));
-    h->built = true;
-    if (topo) h->num_views = 0;  // colour tables are per slot: views must be set again
-    else if (h->num_views > 0) {
+    if (!topo && h->num_views > 0)
         HIPCHK(h, rg::launch_sh_forward(h->g, h->bvh.iperm, h->cams, h->num_views, h->rgb, s));
-    }
+    h->built = true;
     return RG_OK;
 }
 
@@ -351,6 +362,9 @@ int rg_set_views(rg_handle* h, int32_t num_views, const rg_camera* cams, void* s
     if (!h || !cams) return fail(h, RG_ERR_ARG, "null argument");
     if (num_views <= 0) return fail(h, RG_ERR_ARG, "num_views must be positive");
     if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    // the backward's merge key is view * n + slot in 32 bits
+    if ((int64_t)num_views * std::max<int64_t>(h->n, 1) >= (int64_t)INT_MAX)
+        return fail(h, RG_ERR_ARG, "num_views * n must stay below 2^31 - 1 (set fewer views per batch)");
     for (int v = 0; v < num_views; ++v)
         if (cams[v].width <= 0 || cams[v].height <= 0) return fail(h, RG_ERR_ARG, "camera size must be positive");
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -373,50 +387,55 @@ int rg_set_view_colors(rg_handle* h, int32_t view, const float* rgb, void* strea
     return RG_OK;
 }
 
-// Hit-list records: with cfg.replay_words > 0 a forward records, per ray, the lists it gathered;
-// a backward over the same ray set (same arrays, same view, no rg_prepare, rg_set_views or
-// rg_set_config in between) replays them instead of walking the LBVH again.
-static void rec_forward(rg_handle* h, const rg::RayJob& j, rg::DevParams& p) {
+// Hit-list records: with cfg.replay_words > 0 a forward records, per ray, the lists it gathered
+// and returns a token for them; a backward given that token (and nothing in between: no
+// rg_prepare, rg_set_views, rg_set_config or other recording forward) replays them instead of
+// walking the LBVH again.  Each output row's records carry the (view, pixel) they were made
+// for, and the backward replays a row only for that same ray, so a caller that rewrites its ray
+// arrays between the two calls still gets correct gradients (those rays walk the tree).
+static uint64_t rec_forward(rg_handle* h, const rg::RayJob& j, rg::DevParams& p) {
     h->rec_set.valid = false;
     const int words = h->cfg.replay_words;
-    if (words <= 0 || j.num_rays <= 0) return;
+    if (words <= 0 || j.num_rays <= 0) return 0;
     if (h->rec_cap_rays < j.num_rays || h->rec_cap_words < words) {
         h->rec_cap_rays = 0;
         h->rec_cap_words = 0;
         // without the memory the pass runs without records (the backward walks the tree)
         const int64_t rays = (j.num_rays + 63) & ~(int64_t)63;  // whole groups of 64 rays
-        if (dalloc(h, &h->rec, rays * (int64_t)words) != RG_OK || dalloc(h, &h->rec_len, j.num_rays) != RG_OK)
-            return;
+        if (dalloc(h, &h->rec, rays * (int64_t)words) != RG_OK || dalloc(h, &h->rec_len, j.num_rays) != RG_OK ||
+            dalloc(h, &h->rec_id, j.num_rays) != RG_OK)
+            return 0;
         h->rec_cap_rays = j.num_rays;
         h->rec_cap_words = words;
     }
     p.rec = h->rec;
     p.rec_len = h->rec_len;
+    p.rec_id = h->rec_id;
     p.rec_words = words;
     p.rec_mode = 1;
     h->rec_set.valid = true;
+    h->rec_set.token = ++h->tokens_issued;
     h->rec_set.gen = h->generation;
-    h->rec_set.ray_view = j.ray_view;
-    h->rec_set.ray_pixel = j.ray_pixel;
-    h->rec_set.pixels = j.pixels;
     h->rec_set.num_rays = j.num_rays;
-    h->rec_set.view = j.view;
     h->rec_set.words = words;
+    return h->rec_set.token;
 }
 
-static void rec_backward(const rg_handle* h, const rg::RayJob& j, rg::DevParams& p) {
+static void rec_backward(const rg_handle* h, const rg::RayJob& j, uint64_t token, rg::DevParams& p) {
     const auto& r = h->rec_set;
-    if (!r.valid || r.gen != h->generation || r.words != h->cfg.replay_words || r.ray_view != j.ray_view ||
-        r.ray_pixel != j.ray_pixel || r.pixels != j.pixels || r.num_rays != j.num_rays || r.view != j.view)
+    if (token == 0 || !r.valid || r.token != token || r.gen != h->generation || r.words != h->cfg.replay_words ||
+        r.num_rays < j.num_rays)
         return;
     p.rec = h->rec;
     p.rec_len = h->rec_len;
+    p.rec_id = h->rec_id;
     p.rec_words = r.words;
     p.rec_mode = 2;
 }
 
 int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, float* out_rgb, float* out_depth,
-               float* out_opacity, void* stream) {
+               float* out_opacity, uint64_t* out_token, void* stream) {
+    if (out_token) *out_token = 0;
     if (!h || !out_rgb) return fail(h, RG_ERR_ARG, "null argument");
     int rc;
     if ((rc = ready_for_rays(h, view))) return rc;
@@ -428,13 +447,14 @@ int rg_forward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixe
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
     rg::DevParams p = dev_params(h);
-    rec_forward(h, job, p);
+    const uint64_t token = rec_forward(h, job, p);
     HIPCHK(h, rg::launch_forward(dev_scene(h), p, job, fwd_depth(h), out_rgb, out_depth, out_opacity, dev_stats(h), s));
+    if (out_token) *out_token = token;
     return RG_OK;
 }
 
 int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pixels, const float* ray_colors,
-                const float* dL_dC, void* stream) {
+                const float* dL_dC, uint64_t replay_token, void* stream) {
     if (!h || !ray_colors || !dL_dC) return fail(h, RG_ERR_ARG, "null argument");
     int rc;
     if ((rc = ready_for_rays(h, view))) return rc;
@@ -446,7 +466,7 @@ int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pix
     const rg::RayJob job = single_job(h, view, num_rays, pixels);
     const rg::DevGrad gr = dev_grad(h);
     rg::DevParams p = dev_params(h);
-    rec_backward(h, job, p);
+    rec_backward(h, job, replay_token, p);
     HIPCHK(h, rg::launch_backward(dev_scene(h), p, job, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1, gr,
                                   dev_stats(h), s));
     return RG_OK;
@@ -469,7 +489,8 @@ static int batch_job(rg_handle* h, const rg_ray_batch* b, rg::RayJob& j) {
 }
 
 int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float* out_depth, float* out_opacity,
-                     void* stream) {
+                     uint64_t* out_token, void* stream) {
+    if (out_token) *out_token = 0;
     if (!h || !out_rgb) return fail(h, RG_ERR_ARG, "null argument");
     rg::RayJob j;
     int rc;
@@ -477,12 +498,14 @@ int rg_forward_batch(rg_handle* h, const rg_ray_batch* b, float* out_rgb, float*
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     rg::DevParams p = dev_params(h);
-    rec_forward(h, j, p);
+    const uint64_t token = rec_forward(h, j, p);
     HIPCHK(h, rg::launch_forward(dev_scene(h), p, j, fwd_depth(h), out_rgb, out_depth, out_opacity, dev_stats(h), s));
+    if (out_token) *out_token = token;
     return RG_OK;
 }
 
-int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colors, const float* dL_dC, void* stream) {
+int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colors, const float* dL_dC,
+                      uint64_t replay_token, void* stream) {
     if (!h || !ray_colors || !dL_dC) return fail(h, RG_ERR_ARG, "null argument");
     rg::RayJob j;
     int rc;
@@ -491,7 +514,7 @@ int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colo
     HIPCHK(h, rg::launch_reset_counters(dev_stats(h), s));
     const rg::DevGrad gr = dev_grad(h);
     rg::DevParams p = dev_params(h);
-    rec_backward(h, j, p);
+    rec_backward(h, j, replay_token, p);
     HIPCHK(h, rg::launch_backward(dev_scene(h), p, j, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1, gr,
                                   dev_stats(h), s));
     return RG_OK;
@@ -517,9 +540,10 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
     }
     const rg::DevScene sc = dev_scene(h);
     rg::DevParams p = dev_params(h);
-    rec_forward(h, j, p);
+    const uint64_t token = rec_forward(h, j, p);
     HIPCHK(h, rg::launch_forward(sc, p, j, fwd_depth(h), rgb, nullptr, nullptr, dev_stats(h), s));
-    rec_backward(h, j, p);
+    p = dev_params(h);
+    rec_backward(h, j, token, p);
     HIPCHK(h, rg::launch_backward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, target, loss_kind, 0, dev_grad(h),
                                   dev_stats(h), s));
     if (out_loss)
@@ -557,7 +581,7 @@ int rg_scene_box(rg_handle* h, float* out6, void* stream) {
 
 int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
     if (!h || !out) return fail(h, RG_ERR_ARG, "null argument");
-    unsigned long long c[10];
+    unsigned long long c[12];
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIPCHK(h, hipMemcpyAsync(c, h->counters, sizeof(c), hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
@@ -568,8 +592,12 @@ int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
     out->gathers = (int64_t)c[4];
     out->replay_words = (int64_t)c[8];
     out->replay_truncated = (int64_t)c[9];
+    out->dropped = (int64_t)c[10];
+    out->bad_rays = (int64_t)c[11];
     return RG_OK;
 }
+
+int64_t rg_alloc_epoch(const rg_handle* h) { return h ? h->alloc_epoch : -1; }
 
 int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, void* stream) {
     if (!h || !mu || !Mp || !valid) return fail(h, RG_ERR_ARG, "null argument");

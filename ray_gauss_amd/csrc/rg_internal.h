@@ -55,6 +55,7 @@ struct DevParams {
     // written (bit 31: the records stop early).  rec_mode 0 off, 1 forward writes, 2 backward reads.
     unsigned int* rec;
     int* rec_len;
+    int2* rec_id;  // (view, pixel) of the ray whose records row r holds
     int rec_words;
     int rec_mode;
 };
@@ -67,7 +68,8 @@ struct DevGrad {
 };
 
 // Counters (device, 16): 0 K, 1 P, 2 overflow slabs, 3 fallback slabs, 4 gathers, 6 loss (float),
-// 7 debug scratch, 8..15 unused
+// 7 debug scratch, 8 record words, 9 truncated records, 10 dropped by bounds checks, 11 rays with
+// an out-of-range view or pixel, 12..15 unused
 struct DevStats {
     unsigned long long* c;
 };

@@ -42,7 +42,7 @@ constexpr int kMaxSeg = 4096;
 constexpr int kGradVals = 14;
 
 enum CounterIdx { C_K = 0, C_P = 1, C_OVF = 2, C_FALLBACK = 3, C_GATHER = 4, C_LOSS = 6, C_REC_WORDS = 8,
-                  C_REC_FULL = 9 };
+                  C_REC_FULL = 9, C_DROP = 10, C_BAD = 11 };
 
 struct RayOut {
     float C[3];
@@ -73,8 +73,10 @@ __device__ __forceinline__ unsigned int slab_mask(const IList<CAP>& L, int cnt, 
 
 // Sampling quadratic of list entry e (as computed by intersect + refine_sampling at gather time).
 
+// drop: gradient records or replayed hits skipped by a bounds check (an internal error: every
+// parity test asserts zero); bad: rays whose view or pixel index is out of range (skipped).
 struct Counters {
-    unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0, rec_words = 0, rec_full = 0;
+    unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0, rec_words = 0, rec_full = 0, drop = 0, bad = 0;
 };
 
 // FP32 gradient add (device scope, relaxed: the result is read by a later kernel).
@@ -319,7 +321,7 @@ struct GradStage {
 // records were written by other lanes: the workgroup is one wavefront, and the barriers order
 // the LDS writes before these reads and these reads before the next records' writes.
 __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScene& sc, const DevGrad& gr,
-                                            int lane) {
+                                            int lane, unsigned int& dropped) {
     __syncthreads();
     const int j = lane & 15;
     for (int e0 = 0; e0 < cnt; e0 += 4) {
@@ -328,6 +330,8 @@ __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScen
             const int slot = G.slot[e], view = G.view[e];
             if ((unsigned)slot < (unsigned)sc.n && (unsigned)view < (unsigned)gr.num_views)
                 gadd(grad_addr(gr, sc.n, slot, view, j), G.val[e][j]);
+            else if (j == 0)
+                ++dropped;  // never expected: counted so that tests see it
         }
     }
     __syncthreads();
@@ -351,7 +355,8 @@ __device__ __forceinline__ void rs_step(const float* in, float* out, bool upper)
 // bitrev4(l >> 2) summed over 16 lanes, and two more steps over bits 1..0 complete it (17
 // exchanges instead of 14 x 6).  Called by all 64 lanes together.
 __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int view, const DevScene& sc,
-                                                  const DevGrad& gr, int lane, GradStage& G, int& nst) {
+                                                  const DevGrad& gr, int lane, GradStage& G, int& nst,
+                                                  unsigned int& dropped) {
     float a[16], b[8], c[4], d[2];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = k < kGradVals ? v[k] : 0.0f;
@@ -369,7 +374,7 @@ __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int 
         G.view[nst] = view;
     }
     if (++nst == kStage) {
-        stage_flush(G, kStage, sc, gr, lane);
+        stage_flush(G, kStage, sc, gr, lane, dropped);
         nst = 0;
     }
 }
@@ -698,30 +703,6 @@ struct RayRef {
     bool active;
 };
 
-// Maps a work index to (view, pixel, output row).
-__device__ __forceinline__ RayRef ray_ref(const RayJob& job, int64_t r) {
-    RayRef q;
-    q.active = false;
-    q.view = job.view;
-    q.pixel = 0;
-    q.out = r;
-    if (r >= job.num_rays) return q;
-    if (job.ray_view) {
-        q.view = job.ray_view[r];
-        q.pixel = job.ray_pixel[r];
-        q.active = true;
-        return q;
-    }
-    if (job.pixels) {
-        q.pixel = job.pixels[r];
-        q.active = true;
-        return q;
-    }
-    q.pixel = (int)r;
-    q.active = true;
-    return q;
-}
-
 // Tile-ordered enumeration for the implicit full-view mode: work item w -> pixel.
 __device__ __forceinline__ bool tile_pixel(int64_t w, int width, int height, int& pixel) {
     const int64_t tile = w >> 6;
@@ -736,17 +717,17 @@ __device__ __forceinline__ bool tile_pixel(int64_t w, int width, int height, int
 
 __device__ __forceinline__ void flush_counters(const Counters& c, DevStats st) {
     // wave-level reduction, one atomic per counter per wave
-    unsigned int v[7] = {c.K, c.P, c.ovf, c.fallback, c.gathers, c.rec_words, c.rec_full};
-    const int idx[7] = {C_K, C_P, C_OVF, C_FALLBACK, C_GATHER, C_REC_WORDS, C_REC_FULL};
+    unsigned int v[9] = {c.K, c.P, c.ovf, c.fallback, c.gathers, c.rec_words, c.rec_full, c.drop, c.bad};
+    const int idx[9] = {C_K, C_P, C_OVF, C_FALLBACK, C_GATHER, C_REC_WORDS, C_REC_FULL, C_DROP, C_BAD};
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < 9; ++k) {
         unsigned int x = v[k];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         v[k] = x;
     }
     if (wave_lane() == 0) {
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 9; ++k)
             if (v[k]) atomicAdd(&st.c[idx[k]], (unsigned long long)v[k]);
     }
 }
@@ -825,7 +806,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
                                     const float4* __restrict__ rgbv, float cur_tb, unsigned int cur_seed,
                                     float cur_tmin, float cur_tmax,
                                     const float* Aq, const float* Bq, const float* Eq, const float* dL, GradStage& G,
-                                    int& nst, int lane, const DevGrad& gr, int view) {
+                                    int& nst, int lane, const DevGrad& gr, int view, unsigned int& dropped) {
     const float ix = 1.0f / ray.dx, iy = 1.0f / ray.dy, iz = 1.0f / ray.dz;
     const int total = sc.total_nodes;
     bool live = dsc;
@@ -868,7 +849,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             float w[kGradVals];
 #pragma unroll
             for (int k = 0; k < kGradVals; ++k) w[k] = mine ? v[k] : 0.0f;
-            grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst);
+            grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst, dropped);
             rem &= ~__ballot(mine);
         }
         idx = miss;
@@ -941,6 +922,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     for (int i = 0; i < s.dcnt; ++i) {
                         const int slot = (int)*rec_at(rc, s.dpos + i);
                         if ((unsigned)slot < (unsigned)sc.n) direct_hit(slot);
+                        else ++ctr.drop;
                     }
                     s.dense = nh > CAP;
                 } else if (s.direct) {
@@ -983,7 +965,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         // converged: slabs whose hit set overflowed the list scatter from a packet traversal
         if (__ballot(dsc) != 0ULL)
             direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, cg, dL, G,
-                                     nst, lane, gr, view);
+                                     nst, lane, gr, view, ctr.drop);
         // wave-merged scatter over the prepared slabs' list hits
         int e = 0;
         while (true) {
@@ -995,6 +977,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         my = key_base + sl;
                         break;
                     }
+                    ++ctr.drop;
                     ++e;
                 }
             } else if (has) {
@@ -1032,33 +1015,53 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     refine_sampling(g0, g1, g2, g3, ray, I);
                     grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc,
                                         v);
+                } else {
+                    ++ctr.drop;
                 }
                 ++e;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
-            grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst);
+            grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst, ctr.drop);
         }
     }
-    stage_flush(G, nst, sc, gr, lane);
+    stage_flush(G, nst, sc, gr, lane, ctr.drop);
 }
 
 // ------------------------------------------------------------------ kernels ------------------
 
-// Work item -> (view, pixel, output row); for the implicit single-view mode the items are
-// enumerated in 8x8 tiles (num_items is then the padded tile count * 64).
-__device__ __forceinline__ bool resolve(const RayJob& job, int64_t w, RayRef& q) {
+enum RayStatus { RAY_NONE = 0, RAY_OK = 1, RAY_BAD = 2 };
+
+// Work item -> (view, pixel, output row).  RAY_NONE: past the end of the job, or a padding item
+// of the implicit single-view mode (whose items are enumerated in 8x8 tiles, num_items being the
+// padded tile count * 64).  RAY_BAD: a listed view or pixel index outside the views set by
+// rg_set_views; such a ray is skipped (its output row reads NaN) and counted.
+__device__ __forceinline__ int resolve(const RayJob& job, int64_t w, RayRef& q) {
+    q.view = job.view;
+    q.pixel = 0;
+    q.out = w;
+    q.active = false;
     if (job.ray_view || job.pixels) {
-        q = ray_ref(job, w);
-        return q.active;
+        if (w >= job.num_rays) return RAY_NONE;
+        if (job.ray_view) {
+            q.view = job.ray_view[w];
+            q.pixel = job.ray_pixel[w];
+        } else {
+            q.pixel = job.pixels[w];
+        }
+        if ((unsigned)q.view >= (unsigned)job.num_views) return RAY_BAD;
+        const rg_camera& cam = job.cams[q.view];
+        if (q.pixel < 0 || (int64_t)q.pixel >= (int64_t)cam.width * cam.height) return RAY_BAD;
+        q.active = true;
+        return RAY_OK;
     }
+    if (w >= job.num_rays) return RAY_NONE;
     const rg_camera& cam = job.cams[job.view];
     int pixel;
-    if (!tile_pixel(w, cam.width, cam.height, pixel)) return false;
-    q.view = job.view;
+    if (!tile_pixel(w, cam.width, cam.height, pixel)) return RAY_NONE;
     q.pixel = pixel;
     q.out = pixel;
     q.active = true;
-    return true;
+    return RAY_OK;
 }
 
 // XCD-aware block order: consecutive work blocks land on the same XCD's L2.
@@ -1078,7 +1081,20 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB 
     const int64_t item = block_item(blockIdx.x, nb) * kWave + lane;
     Counters ctr;
     RayRef q;
-    if (item < num_items && resolve(job, item, q)) {
+    const int rs = item < num_items ? resolve(job, item, q) : (int)RAY_NONE;
+    if (rs == RAY_BAD) {
+        ++ctr.bad;
+        const float nan = __builtin_nanf("");
+        out_rgb[3 * q.out + 0] = nan;
+        out_rgb[3 * q.out + 1] = nan;
+        out_rgb[3 * q.out + 2] = nan;
+        if (out_depth) out_depth[q.out] = nan;
+        if (out_opac) out_opac[q.out] = nan;
+        if constexpr (RM == 1) {
+            p.rec_len[q.out] = 0;
+            p.rec_id[q.out] = make_int2(-1, -1);
+        }
+    } else if (rs == RAY_OK) {
         const rg_camera cam = job.cams[q.view];
         unsigned int seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
         const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
@@ -1091,6 +1107,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB 
         forward_ray<SPB, CAP, RNG, RM>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o, rc);
         if constexpr (RM == 1) {
             p.rec_len[q.out] = rc.pos | (rc.full ? (int)0x80000000u : 0);
+            p.rec_id[q.out] = make_int2(q.view, q.pixel);  // the backward replays only the same ray
             ctr.rec_words += rc.pos;
             ctr.rec_full += rc.full ? 1u : 0u;
         }
@@ -1126,7 +1143,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
     float loss = 0.0f;
     int view = 0;
     Rec rc;
-    if (item < num_items && resolve(job, item, q)) {
+    const int rs = item < num_items ? resolve(job, item, q) : (int)RAY_NONE;
+    if (rs == RAY_BAD) ++ctr.bad;
+    if (rs == RAY_OK) {
         Cf[0] = ray_colors[3 * q.out];
         Cf[1] = ray_colors[3 * q.out + 1];
         Cf[2] = ray_colors[3 * q.out + 2];
@@ -1151,8 +1170,13 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
             active = true;
             view = q.view;
             if constexpr (RM == 2) {
-                rc.w = rec_base(p.rec, q.out, p.rec_words);
-                rc.end = min(p.rec_len[q.out] & 0x7fffffff, p.rec_words);
+                // records of this output row are replayed only when the forward wrote them for
+                // the same (view, pixel); otherwise the ray walks the tree
+                const int2 id = p.rec_id[q.out];
+                if (id.x == q.view && id.y == q.pixel) {
+                    rc.w = rec_base(p.rec, q.out, p.rec_words);
+                    rc.end = min(p.rec_len[q.out] & 0x7fffffff, p.rec_words);
+                }
             }
             const rg_camera cam = job.cams[q.view];
             seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
@@ -1174,7 +1198,16 @@ __global__ void __launch_bounds__(kWave) k_debug_hits(DevScene sc, DevParams p, 
                                                       float* rd, float* te, float* tx) {
     const int64_t r = (int64_t)blockIdx.x * kWave + threadIdx.x;
     if (r >= job.num_rays) return;
-    const RayRef q = ray_ref(job, r);
+    RayRef q;
+    if (job.pixels || job.ray_view) {
+        if (resolve(job, r, q) != RAY_OK) {
+            counts[r] = 0;
+            return;
+        }
+    } else {  // no pixel list: ray r is pixel r (row-major)
+        q.view = job.view;
+        q.pixel = (int)r;
+    }
     const rg_camera cam = job.cams[q.view];
     unsigned int seed = tea4(p.iteration, (unsigned int)q.pixel);
     const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, p.jitter, &seed);

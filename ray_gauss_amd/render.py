@@ -109,6 +109,10 @@ class RenderOptixFunction(torch.autograd.Function):
                 print(" ############ Number slabs with max gaussians exceeded : ", ovf)
         settings.gas = caster
         ctx.settings = settings
+        ctx.context = settings.context
+        ctx.max_prim_slice = int(settings.max_prim_slice or 0)
+        ctx.exact_sh_grad = bool(settings.exact_sh_grad)
+        ctx.token = caster.last_token
         ctx.caster = caster
         ctx.cam = cam
         ctx.generation = caster.generation
@@ -124,18 +128,18 @@ class RenderOptixFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dloss_dray_colors, *unused):
         (positions, scales, quats, densities, color_features, sg, sharp, axis, ray_colors) = ctx.saved_tensors
-        caster = ctx.caster
         L = ctx.num_sg
-        if caster.params != ctx.params:
-            # another call (e.g. a later iteration) changed the handle's settings: restore this one's
-            caster.params = ctx.params
-            caster.set_tune(caster.tune)
+        # another call (e.g. a later iteration, or an inference render) may have changed the
+        # handle's settings: restore this call's through the context, which keeps its own record
+        # of the settings in force current
+        caster = ctx.context.caster_for(ctx.params, ctx.max_prim_slice, ctx.exact_sh_grad)
         if caster.generation != ctx.generation:
             # another forward used this handle in between: restore this call's state
             caster.prepare(positions, scales, quats, densities, color_features, sg if L else None,
                            sharp if L else None, axis if L else None, rebuild=True)
             caster.set_views([ctx.cam])
-        caster.backward(0, None, ray_colors, dloss_dray_colors.contiguous())
+        # the recorded hit lists are used only if nothing replaced them since this forward
+        caster.backward(0, None, ray_colors, dloss_dray_colors.contiguous(), ctx.token)
         g = caster.finish_grads()
         if L:
             gsg, gsh, gax = g["sg"], g["sharpness"], g["lobe_axis"]

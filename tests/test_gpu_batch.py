@@ -67,7 +67,7 @@ def test_fused_train_equals_forward_backward(blobs):
     gen = torch.Generator(device=rv.device).manual_seed(0)
     target = torch.rand((rv.shape[0], 3), generator=gen, device=rv.device)
     rgb, _, _ = rc.forward_batch(rv, rp)
-    rc.backward_batch(rv, rp, rgb, 2.0 * (rgb - target))
+    rc.backward_batch(rv, rp, rgb, 2.0 * (rgb - target), rc.last_token)
     g_sep = rc.finish_grads()
     loss = torch.zeros(1, device=rv.device)
     out = torch.empty_like(rgb)
@@ -113,19 +113,29 @@ def test_replayed_hit_lists_give_the_same_step(blobs, gpu_device, words):
 
 
 def test_replay_only_for_the_recorded_rays(blobs, gpu_device):
-    """forward_batch records, backward_batch replays only over the same ray arrays with nothing
-    changed in between: a backward over other rays, or after set_views, walks the tree and gives
-    the same gradients as without records."""
+    """forward_batch records and returns a token; backward_batch replays only when given that
+    token with nothing changed in between: after another recording forward or set_views the
+    token is stale and the backward walks the tree, giving the same gradients as without
+    records.  A caller that rewrites its ray arrays in place between the two calls still gets
+    the walked gradients for the rewritten rays (each output row's records name their ray)."""
     sc, prm, cams, pixels, rc0, t0, rv, rp, offs = blobs
     gen = torch.Generator(device=gpu_device).manual_seed(3)
     target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
     rv2, rp2 = rv.flip(0).contiguous(), rp.flip(0).contiguous()
 
-    def run(words, between):
+    def run(words, between, rewrite=False):
         rc, _ = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(replay_words=words))
-        rgb, _, _ = rc.forward_batch(rv, rp)
+        rv_, rp_ = rv.clone(), rp.clone()
+        rgb, _, _ = rc.forward_batch(rv_, rp_)
+        tok = rc.last_token
+        assert (tok != 0) == (words > 0)
         between(rc)
-        rc.backward_batch(rv, rp, rgb, 2.0 * (rgb - target))
+        if rewrite:  # rows 0, 3, 6, ... now name the next pixel of their own view
+            npx = torch.tensor([c["width"] * c["height"] for c in cams], device=rv.device)[rv.long()]
+            rp_[::3] = ((rp[::3].long() + 1) % npx[::3]).int()
+        rc.backward_batch(rv_, rp_, rgb, 2.0 * (rgb - target), tok)
+        st = rc.stats()
+        assert st["dropped"] == 0 and st["bad_rays"] == 0
         return {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}
 
     ref = run(0, lambda rc: None)
@@ -133,6 +143,11 @@ def test_replay_only_for_the_recorded_rays(blobs, gpu_device):
         g = run(512, between)
         for k, v in ref.items():
             assert max_rel(g[k], v) <= 1e-5, k
+    ref_rw = run(0, lambda rc: None, rewrite=True)
+    g_rw = run(512, lambda rc: None, rewrite=True)
+    for k, v in ref_rw.items():
+        assert max_rel(g_rw[k], v) <= 1e-5, k
+    assert max_rel(ref_rw["densities"], ref["densities"]) > 1e-3  # the rewrite does change the rays
 
 
 def test_batch_backward_parity_with_oracle(blobs):
@@ -151,7 +166,8 @@ def test_batch_backward_parity_with_oracle(blobs):
         og = O.full_backward(prm, sc, cams[v]["eye"], box[0], box[1], cams[v],
                              rgb[offs[v]:offs[v + 1]].cpu().numpy().astype(np.float64), d, pv)
         total = og if total is None else {k: total[k] + og[k] for k in og}
-    rc.backward_batch(rv, rp, rgb, torch.from_numpy(dL.astype(np.float32)).to(rv.device))
+    rc.backward_batch(rv, rp, rgb, torch.from_numpy(dL.astype(np.float32)).to(rv.device), rc.last_token)
+    assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis"):
         e = max_rel(g[k].cpu().numpy(), total[k])
@@ -223,7 +239,7 @@ def test_dropin_autograd_boundary(gpu_device):
     rc.set_views([wl.cameras[0].launch_record()])
     rgb, _, _ = rc.forward(0)
     assert torch.equal(rgb, img.detach())
-    rc.backward(0, None, rgb, 2 * (rgb - target))
+    rc.backward(0, None, rgb, 2 * (rgb - target), rc.last_token)
     g = rc.finish_grads()
     assert max_rel(dens.grad.cpu().numpy(), g["densities"].cpu().numpy()) <= 1e-5
     assert max_rel(pos.grad.cpu().numpy(), g["positions"].cpu().numpy()) <= 1e-5

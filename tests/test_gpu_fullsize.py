@@ -38,7 +38,8 @@ def test_bvh_and_crop_parity(big):
     assert (diff > 1e-4).mean() <= 0.05
     # gradients on the crop, fragile rays masked on both sides
     dL = np.random.default_rng(3).normal(0, 1, (len(px), 3)) * ok[:, None]
-    rc.backward(v, pxt, rgb, torch.from_numpy(dL.astype(np.float32)).cuda())
+    rc.backward(v, pxt, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(wl.params, wl.scene, cams[v]["eye"], box[0], box[1], cams[v],

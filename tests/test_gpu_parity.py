@@ -95,7 +95,8 @@ def test_backward_parity_small(small):
     ok = (ref["fragile"] == 0).astype(np.float64)
     target = np.random.default_rng(1).uniform(0, 1, (4096, 3))
     dL = 2.0 * (rgb.cpu().numpy().astype(np.float64) - target) * ok[:, None]
-    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda())
+    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(wl.params, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64),
@@ -136,6 +137,7 @@ def test_traversal_modes_agree(gpu_device):
         rgb, _, _ = rc.forward(0, px)
         st = rc.stats()
         rc.backward(0, px, rgb, 2.0 * (rgb - target))
+        assert rc.stats()["dropped"] == 0
         g = {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}
         out[mode] = (rgb.cpu().numpy(), g, st)
     assert out[False][2]["fallback_slabs"] > 0  # the direct (overflow) path is exercised
@@ -158,7 +160,8 @@ def test_blobs_crop_parity(gpu_device, builder):
     err = np.abs(rgb.cpu().numpy() - ref["rgb"])[ok].max()
     assert err <= IMG_TOL, err
     dL = np.random.default_rng(2).normal(0, 1, (px.shape[0], 3)) * ok[:, None]
-    rc.backward(0, px, rgb, torch.from_numpy(dL.astype(np.float32)).cuda())
+    rc.backward(0, px, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(wl.params, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64),
@@ -192,7 +195,8 @@ def test_random_sampling_parity(gpu_device, jitter, rnd):
     assert np.abs(ref["rgb"] - base["rgb"]).max() > 1e-4  # the settings do change the image
     target = np.random.default_rng(3).uniform(0, 1, (4096, 3))
     dL = 2.0 * (rgb.cpu().numpy().astype(np.float64) - target) * ok[:, None]
-    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda())
+    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(prm, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64), dL)

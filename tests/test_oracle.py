@@ -460,3 +460,84 @@ def test_sample_offsets_keep_the_single_gaussian_closed_form():
         base = O.render(scenes.RenderParams(**{**prm.__dict__, "rnd_sample": 0}), one, rgb, bmin, bmax, cam, px, 1)
         assert not np.allclose(out["rgb"], base["rgb"], rtol=0, atol=1e-9)
         assert np.array_equal(out["rgb"], out2["rgb"]) == (mode == 2)
+
+
+# ---- spherical-Gaussian colour term and depth: closed forms ----------------------------------
+
+def test_sg_term_closed_form():
+    """rgb - 0.5 - SH = sum_l a[c,l] exp(lambda_l (nu_l . v - 1)) per channel, v the unit view
+    direction (utils/sh_utils.cu:43-54), with layouts sg (n,3,L) = [g, channel, lobe], sharpness
+    (n,L), lobe_axis (n,L,3).  Lobe axes at v, -v and perpendicular to v make the three terms
+    exactly a, a exp(-2 lambda) and a exp(-lambda); every (channel, lobe) amplitude and every
+    lobe's sharpness is distinct, so a transposed index or a wrong exponent changes the result."""
+    eye = np.array([0.5, -2.0, 0.25], np.float32)
+    pos = np.array([[0.25, 1.0, -0.5]], np.float32)
+    v = pos[0].astype(np.float64) - eye.astype(np.float64)
+    v /= np.linalg.norm(v)
+    perp = np.cross(v, [0.0, 0.0, 1.0])
+    perp /= np.linalg.norm(perp)
+    axes = np.stack([v, -v, perp])[None]  # (1, L=3, 3)
+    lam = np.array([[1.5, 0.75, 2.5]])
+    a = np.array([[[0.11, -0.23, 0.37], [0.41, 0.05, -0.29], [-0.17, 0.31, 0.13]]])  # [g, c, l]
+    sh = np.zeros((1, 3, 16))
+    sh[0, :, 0] = [0.2, -0.1, 0.05]
+    sc = make_scene(pos, [[1, 1, 1]], [[1, 0, 0, 0]], [1.0], sh=sh, sg=a, sharp=lam, axis=axes, deg=3, L=3)
+    got = O.sh_forward(sc, eye)[0] - 0.5 - 0.28209479177387814 * sc.sh[0, :, 0].astype(np.float64)
+    # the float32 axes are not exactly +-v / perpendicular: evaluate nu.v with them
+    ax32 = sc.lobe_axis[0].astype(np.float64)
+    lam32 = sc.sharpness[0].astype(np.float64)
+    a32 = sc.sg[0].astype(np.float64)
+    dots = ax32 @ v
+    assert abs(dots[0] - 1) < 1e-6 and abs(dots[1] + 1) < 1e-6 and abs(dots[2]) < 1e-6
+    special = np.exp(lam32 * (np.array([1.0, -1.0, 0.0]) - 1.0))  # 1, exp(-2 lam), exp(-lam)
+    np.testing.assert_allclose(got, a32 @ special, rtol=0, atol=2e-6)
+    exact = a32 @ np.exp(lam32 * (dots - 1.0))
+    np.testing.assert_allclose(got, exact, rtol=0, atol=1e-12)
+    # the transposed layout a[g, l, c] gives a different colour
+    assert np.abs(a32.T @ special - got).max() > 1e-2
+
+
+def test_depth_single_gaussian_analytic_sequence():
+    """depth = sum_i T_i alpha_i t_i (cuda_gui/gaussians_aabb.cu:400-417) for one Gaussian centred
+    on the optical axis, from the analytic density sequence sigma_i = sigma
+    exp(-(t_i - t_c)^2 / (2 s_y^2)) (zero where sigma_i <= thr) on the FP32 sample grid
+    t_i = tb + i dt, tb stepping from the scene-box entry by float32(dt * 8); t_c = |mu - eye|.
+    The quaternion (1/2, 1/2, 1/2, 1/2) (exactly unit in float32) maps local axis 0 to world y,
+    the viewing direction, so s_y = s_0; the transposed rotation would pick s_2 instead.  The
+    oracle evaluates the weights through the general ellipsoid machinery, so the check pins it,
+    the slab loop and the depth sum.  As the density goes to zero, depth / opacity tends to t_c
+    by symmetry."""
+    s3 = [0.1, 0.15, 0.2]
+    cam = look_at_camera([0.0, -3.0, 0.0], width=1, height=1, fovx_deg=10.0).launch_record()
+    for sigma in (0.6, 5.0):
+        sc = make_scene([[0, 0, 0]], [s3], [[0.5, 0.5, 0.5, 0.5]], [sigma])
+        params = RenderParams(dt=0.002, t_eps=0.0)
+        out, _ = _render_one(sc, params, cam)
+        bmin, bmax = O.scene_box(sc, params.sigma_thr)
+        te, tx = O.ray_interval(cam, bmin, bmax, params.t_near, params.t_far)
+        o, d = O.rays(cam)
+        assert np.array_equal(d[0], np.array([0, 1, 0], np.float32))
+        tc = float(np.linalg.norm(-o[0].astype(np.float64)))
+        sy = float(sc.scales[0, 0])
+        dt = np.float32(params.dt)
+        slab = np.float32(dt * np.float32(8))
+        sig32 = float(sc.densities[0])
+        T, depth = 1.0, 0.0
+        tb = np.float32(te[0])
+        while tb < tx[0]:
+            for i in range(8):
+                t = float(np.float32(tb + np.float32(np.float32(i) * dt)))
+                si = sig32 * math.exp(-((t - tc) ** 2) / (2 * sy ** 2))
+                si = si if si > float(np.float32(0.1)) else 0.0
+                alpha = 1.0 - math.exp(-si * float(dt))
+                depth += T * alpha * t
+                T *= 1.0 - alpha
+            tb = np.float32(tb + slab)
+        assert 1 - T > 1e-3
+        assert abs(out["depth"][0] - depth) < 1e-11 * max(1.0, depth), (sigma, out["depth"][0], depth)
+        assert abs(out["opacity"][0] - (1 - T)) < 1e-11
+    # low-density limit: the expected termination distance is the centre
+    sc = make_scene([[0, 0, 0]], [s3], [[1, 0, 0, 0]], [0.2])
+    out, _ = _render_one(sc, RenderParams(dt=0.0005, t_eps=0.0), cam)
+    assert out["opacity"][0] > 0
+    assert abs(out["depth"][0] / out["opacity"][0] - 3.0) < 2e-3
