@@ -301,6 +301,16 @@ int64_t rg_knn_workspace_bytes(int64_t n);
 int rg_knn_mean_dist(const float* points, int64_t n, float* out, void* workspace, int64_t workspace_bytes,
                      void* stream);
 
+/* rg_knn_mean_dist without a workspace argument and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The workspace is a static 64 MiB
+ * device array that every GPU the library runs on holds until the process ends (one copy per
+ * device; the calling thread's current device must be the stream's).  n is limited to what fits
+ * it (about 1.6M points: the message of RG_ERR_ARG names the exact limit); larger n returns
+ * RG_ERR_ARG.  Calls that overlap on one GPU (two streams, two threads) share the array and race:
+ * serialise them. */
+int rg_knn_mean_dist_noalloc(const float* points, int64_t n, float* out, void* stream);
+
 /* One camera of the 3D filter, prepared on the host from the reference's camera:
  * R (row-major, as camera.R: x_cam = x R + T), T, fx = fov2focal(FoVx, W), fy, cx = W/2, cy = H/2,
  * and the screen window [-0.15 W, 1.15 W] x [-0.15 H, 1.15 H], all rounded to float. */

@@ -715,6 +715,28 @@ int rg_knn_mean_dist(const float* points, int64_t n, float* out, void* workspace
     return RG_OK;
 }
 
+int rg_knn_mean_dist_noalloc(const float* points, int64_t n, float* out, void* stream) {
+    if (n < 0 || n > kMaxUtilRows) return fail_free(RG_ERR_ARG, "rg_knn_mean_dist_noalloc: n out of range");
+    if (n == 0) return RG_OK;
+    if (!points || !out) return fail_free(RG_ERR_ARG, "rg_knn_mean_dist_noalloc: null argument");
+    const int64_t cap = rg::knn_scratch_bytes();
+    if (rg::knn_workspace_bytes(n) > cap) {
+        // largest n whose workspace fits (the workspace grows monotonically with n)
+        int64_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            (rg::knn_workspace_bytes(mid) <= cap ? lo : hi) = mid;
+        }
+        return fail_free(RG_ERR_ARG, "rg_knn_mean_dist_noalloc: n = " + std::to_string(n) + " needs " +
+                                         std::to_string(rg::knn_workspace_bytes(n)) + " bytes of scratch; the static " +
+                                         "scratch holds " + std::to_string(cap) + " bytes (n <= " + std::to_string(lo) +
+                                         "): use rg_knn_mean_dist with a workspace");
+    }
+    const hipError_t e = rg::launch_knn_mean_dist_noalloc(points, n, out, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_free(RG_ERR_HIP, std::string("rg_knn_mean_dist_noalloc: ") + hipGetErrorString(e));
+    return RG_OK;
+}
+
 int rg_filter_3d(const float* points, int64_t n, const rg_filter_camera* cams, int32_t num_cams, float inv_focal,
                  float* out, void* workspace, int64_t workspace_bytes, void* stream) {
     if (n < 0 || n > kMaxUtilRows || num_cams < 0) return fail_free(RG_ERR_ARG, "rg_filter_3d: size out of range");

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
 
 #include "rg_internal.h"
 
@@ -554,6 +555,54 @@ hipError_t launch_knn_mean_dist(const float* points, int64_t n, float* out, void
     hipLaunchKernelGGL(k_knn_group_box, dim3(ns), dim3(kB), 0, s, nb, w.bbox, 1, w.sbox);
     hipLaunchKernelGGL(k_knn_search, dim3(nb), dim3(kB), 0, s, N, w.sp, w.vals[cur], nb, w.bbox, ns, w.sbox, out);
     return hipGetLastError();
+}
+
+// ---- kNN without a workspace argument ---------------------------------------------------------
+//
+// For callers that cannot allocate device memory (code under stream capture, serving loops):
+// the workspace of launch_knn_mean_dist lives in this file-scope array instead.  Every GPU the
+// library runs on holds its own copy until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): it asks the runtime for the
+// array's device address once per device, with that device current, and uses that pointer.  The
+// kernels of one call fully write every scratch word they later read, so nothing needs clearing
+// between calls; two calls that overlap on one GPU share the array and race.
+constexpr int64_t kKnnScratchBytes = (int64_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kKnnScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+}  // namespace
+
+int64_t knn_scratch_bytes() { return kKnnScratchBytes; }
+
+// Device address of this device's scratch array, looked up on the first call per device.
+static hipError_t knn_scratch_ptr(void** out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_knn_mean_dist_noalloc(const float* points, int64_t n, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (knn_workspace_bytes(n) > kKnnScratchBytes) return hipErrorInvalidValue;  // checked by the caller too
+    void* ws = nullptr;
+    const hipError_t e = knn_scratch_ptr(&ws);
+    if (e != hipSuccess) return e;
+    return launch_knn_mean_dist(points, n, out, ws, s);
 }
 
 hipError_t launch_filter_3d(const float* points, int64_t n, const rg_filter_camera* cams_dev, int V, float inv_focal,

@@ -158,6 +158,9 @@ hipError_t launch_project(const rg_projection& pr, hipStream_t s);
 // ---- launchers (densify.hip) -------------------------------------------------------------------
 int64_t knn_workspace_bytes(int64_t n);
 hipError_t launch_knn_mean_dist(const float* points, int64_t n, float* out, void* ws, hipStream_t s);
+// The same with the workspace in a static per-device scratch array of knn_scratch_bytes() bytes.
+int64_t knn_scratch_bytes();
+hipError_t launch_knn_mean_dist_noalloc(const float* points, int64_t n, float* out, hipStream_t s);
 hipError_t launch_filter_3d(const float* points, int64_t n, const rg_filter_camera* cams_dev, int V, float inv_focal,
                             float* out, unsigned int* dmax_bits, hipStream_t s);
 hipError_t launch_accum_grad(const float* g, int64_t n, float* accum, float* num, float* gnv, hipStream_t s);
