@@ -212,9 +212,11 @@ int rg_get_stats(rg_handle* h, rg_stats* out, void* stream);
  * unchanged.  No reference counterpart (the reference captures no graphs). */
 int64_t rg_alloc_epoch(const rg_handle* h);
 
-/* Debug/verification: the FP32 predicate parameters of the bounding ellipsoids in original
- * order -- mu (n,3), Mp (n,9) rows of S'^-1 R^T, valid (n) -- exactly as the kernels use them. */
-int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, void* stream);
+/* Debug/verification: the FP32 per-Gaussian records in original order, exactly as the kernels use
+ * them -- mu (n,3), Mp (n,9) rows of S'^-1 R^T, valid (n), and (optional, may be NULL) hlog (n) =
+ * h = ln|sigma / thr|, the scale of the sampling exponent -- so that a checker can take the same
+ * FP32 decisions (hit sets, sample thresholds) as the kernels. */
+int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, float* hlog, void* stream);
 
 /* Debug/verification: per-slab hit sets found by the LBVH traversal for num_rays rays of a view,
  * enumerating slabs from tenter to texit without transmittance exit.  pairs (num_rays,

@@ -29,7 +29,32 @@ class Params(C.Structure):
     _fields_ = [("dt", C.c_float), ("spb", C.c_int32), ("t_eps", C.c_float), ("sigma_thr", C.c_float),
                 ("t_near", C.c_float), ("t_far", C.c_float), ("white_bg", C.c_int32),
                 ("exact_sh_grad", C.c_int32), ("jitter", C.c_int32), ("rnd_sample", C.c_int32),
-                ("iteration", C.c_uint32)]
+                ("iteration", C.c_uint32), ("eps_w", C.c_float)]
+
+# Fragility margin on sigma*w / thr: a sample decision whose exact value lies within EPS_W of the
+# threshold may be decided either way by an FP32 evaluation.  tests/test_oracle.py measures the
+# FP32 error of both the reference's evaluation and the HIP path's closed form near the threshold
+# (orc_sigmaw_fp32_error) and asserts EPS_W is at least twice the largest one seen.
+EPS_W = 3e-4
+
+
+class RecordS(C.Structure):
+    _fields_ = [("mu", C.c_void_p), ("Mp", C.c_void_p), ("valid", C.c_void_p), ("h", C.c_void_p)]
+
+
+class _RecordHolder:
+    """The GPU's FP32 per-Gaussian records (RayCaster.export_record()) kept alive for C."""
+
+    def __init__(self, rec):
+        self.arrays = [np.ascontiguousarray(rec["mu"], np.float32), np.ascontiguousarray(rec["Mp"], np.float32),
+                       np.ascontiguousarray(rec["valid"], np.int32), np.ascontiguousarray(rec["h"], np.float32)]
+        r = RecordS()
+        r.mu, r.Mp, r.valid, r.h = (a.ctypes.data for a in self.arrays)
+        self.s = r
+
+
+def _rec_ptr(holder):
+    return None if holder is None else C.byref(holder.s)
 
 
 class SceneS(C.Structure):
@@ -95,6 +120,7 @@ def params_struct(prm, exact_sh_grad=False) -> Params:
     p.jitter = int(getattr(prm, "jitter", 0))
     p.rnd_sample = int(getattr(prm, "rnd_sample", 0))
     p.iteration = int(getattr(prm, "iteration", 0)) & 0xFFFFFFFF
+    p.eps_w = float(getattr(prm, "eps_w", EPS_W))
     return p
 
 
@@ -204,21 +230,60 @@ def scene_box(scene, sigma_thr=0.1):
     return lo32.astype(np.float32), hi32.astype(np.float32)
 
 
-def render(prm, scene, rgb, bmin, bmax, cam_rec, pixels=None, threads=None):
+def render(prm, scene, rgb, bmin, bmax, cam_rec, pixels=None, threads=None, record=None):
+    """Forward render of one view.  record (the GPU's export_record()) switches to the
+    decision-matched mode: hit sets and sample thresholds decided in FP32 from those records as
+    the HIP path decides them, every value still FP64; 'fragile' then only flags decisions within
+    1e-6 of the threshold (native vs host exp2), early exits and origin-inside rays."""
     h = _SceneHolder(scene)
+    rh = None if record is None else _RecordHolder(record)
     n = cam_rec["width"] * cam_rec["height"] if pixels is None else len(pixels)
     px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
     out = {
         "rgb": np.empty((n, 3)), "depth": np.empty(n), "opacity": np.empty(n),
         "fragile": np.empty(n, np.int32), "K": np.empty(n, np.int64), "P": np.empty(n, np.int64),
-        "maxhits": np.empty(n, np.int64), "NC": np.empty(n, np.int64),
+        "maxhits": np.empty(n, np.int64), "NC": np.empty(n, np.int64), "nfrag": np.empty(n, np.int64),
+        "cmax": np.empty(n),
     }
     ps = params_struct(prm)
     cs = camera_struct(cam_rec)
     lib().orc_render(C.byref(ps), C.byref(h.s), _p(_f64(rgb)), _p(_f32(bmin)), _p(_f32(bmax)), C.byref(cs),
                      C.c_int64(n), _p(px), _p(out["rgb"]), _p(out["depth"]), _p(out["opacity"]), _p(out["fragile"]),
-                     _p(out["K"]), _p(out["P"]), _p(out["maxhits"]), _p(out["NC"]), C.c_int(nthreads(threads)))
+                     _p(out["K"]), _p(out["P"]), _p(out["maxhits"]), _p(out["NC"]), _p(out["nfrag"]),
+                     _p(out["cmax"]), _rec_ptr(rh), C.c_int(nthreads(threads)))
     return out
+
+
+# kinds of fragility (render()["fragile"] bits)
+FRAG_SAMPLE, FRAG_EXIT, FRAG_QUIRK = 1, 2, 4
+
+
+def fragile_bound(prm, out):
+    """Per-ray bound on |C_fp32 - C_oracle| that the ray's near-threshold decisions allow: each
+    counted sample decision can move the colour by at most 4 thr (1 + eps_w) dt max(cmax, 1) (its
+    density changes by at most thr (1 + eps_w); see the oracle's header note), a transmittance check
+    within rounding of T_eps by at most 2.002 T_eps max(cmax, 1) (one slab more or less); plus the
+    1e-4 tolerance of every ray.  Rays whose origin-inside decision is fragile get +inf (the
+    decision adds or removes a whole Gaussian)."""
+    eps_w = float(getattr(prm, "eps_w", EPS_W))
+    c = np.maximum(out["cmax"], 1.0)
+    b = 1e-4 + out["nfrag"] * 4.0 * float(np.float32(prm.sigma_thr)) * (1.0 + eps_w) * float(np.float32(prm.dt)) * c
+    b = b + ((out["fragile"] & FRAG_EXIT) != 0) * 2.002 * float(np.float32(prm.t_eps)) * c
+    return np.where((out["fragile"] & FRAG_QUIRK) != 0, np.inf, b)
+
+
+def sigmaw_fp32_error(prm, scene, bmin, bmax, cam_rec, pixels=None, band=1e-2, threads=None):
+    """Largest relative FP32 error of sigma*w near the threshold (within `band`): the reference's
+    FP32 evaluation and the HIP path's closed form, against FP64; and the number of samples."""
+    h = _SceneHolder(scene)
+    n = cam_rec["width"] * cam_rec["height"] if pixels is None else len(pixels)
+    px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+    out = np.zeros(3)
+    ps = params_struct(prm)
+    cs = camera_struct(cam_rec)
+    lib().orc_sigmaw_fp32_error(C.byref(ps), C.byref(h.s), _p(_f32(bmin)), _p(_f32(bmax)), C.byref(cs),
+                                C.c_int64(n), _p(px), C.c_double(band), _p(out), C.c_int(nthreads(threads)))
+    return {"reference": float(out[0]), "closed_form": float(out[1]), "samples": int(out[2])}
 
 
 def render_literal(prm, scene, rgb, bmin, bmax, cam_rec, pixels=None, threads=None):
@@ -234,10 +299,11 @@ def render_literal(prm, scene, rgb, bmin, bmax, cam_rec, pixels=None, threads=No
     return out
 
 
-def backward(prm, scene, rgb, bmin, bmax, cam_rec, ray_colors, dLdC, pixels=None, threads=None):
+def backward(prm, scene, rgb, bmin, bmax, cam_rec, ray_colors, dLdC, pixels=None, threads=None, record=None):
     """Ray-caster backward for one view; returns gradients w.r.t. positions, scales, quats,
-    densities and the per-Gaussian colour (rgb) -- before the SH/SG chain."""
+    densities and the per-Gaussian colour (rgb) -- before the SH/SG chain.  record: as in render()."""
     h = _SceneHolder(scene)
+    rh = None if record is None else _RecordHolder(record)
     n = scene.n
     nr = cam_rec["width"] * cam_rec["height"] if pixels is None else len(pixels)
     px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
@@ -247,17 +313,18 @@ def backward(prm, scene, rgb, bmin, bmax, cam_rec, ray_colors, dLdC, pixels=None
     cs = camera_struct(cam_rec)
     lib().orc_backward(C.byref(ps), C.byref(h.s), _p(_f64(rgb)), _p(_f32(bmin)), _p(_f32(bmax)), C.byref(cs),
                        C.c_int64(nr), _p(px), _p(_f64(ray_colors)), _p(_f64(dLdC)), _p(g["positions"]),
-                       _p(g["scales"]), _p(g["quats"]), _p(g["densities"]), _p(g["rgb"]), C.c_int(nthreads(threads)))
+                       _p(g["scales"]), _p(g["quats"]), _p(g["densities"]), _p(g["rgb"]), _rec_ptr(rh),
+                       C.c_int(nthreads(threads)))
     return g
 
 
 def full_backward(prm, scene, eye, bmin, bmax, cam_rec, ray_colors, dLdC, pixels=None, exact_sh=False,
-                  threads=None):
+                  threads=None, record=None):
     """Gradients of all nine boundary inputs, as RenderOptixFunction.backward assembles them
     (render_optix.py:158-190): the ray-caster terms plus the SH/SG chain and the position term
     through the view direction."""
     rgb = sh_forward(scene, eye, threads)
-    g = backward(prm, scene, rgb, bmin, bmax, cam_rec, ray_colors, dLdC, pixels, threads)
+    g = backward(prm, scene, rgb, bmin, bmax, cam_rec, ray_colors, dLdC, pixels, threads, record)
     s = sh_backward(scene, eye, g["rgb"], exact_sh, threads)
     return {"positions": g["positions"] + s["positions"], "scales": g["scales"], "quats": g["quats"],
             "densities": g["densities"], "sh": s["sh"], "sg": s["sg"], "sharpness": s["sharpness"],

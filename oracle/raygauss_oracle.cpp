@@ -80,6 +80,7 @@ struct orc_params {
     int32_t jitter;         // 1: sub-pixel jitter of the ray (forward/gaussians_aabb.cu:409-418)
     int32_t rnd_sample;     // 0: t_i = tb + i dt; 1: + u_i dt, u_i random; 2: + dt/2 (:473-482)
     uint32_t iteration;     // seeds the per-ray generator with tea<4>(iteration, pixel) (:526)
+    float eps_w;            // fragility margin on sigma*w / thr (<= 0: the default kEpsW)
 };
 
 struct orc_scene {
@@ -94,6 +95,18 @@ struct orc_scene {
     const float* sg;      // (n,3,L)
     const float* lam;     // (n,L)
     const float* axis;    // (n,L,3)
+};
+
+// The HIP path's FP32 per-Gaussian records (rg_export_predicate), original order.  Given to
+// orc_render / orc_backward, they switch the oracle to decision-matched mode: every discrete
+// decision (slab hit sets, sample thresholds sigma*w > thr) is taken in FP32 from these records
+// with the HIP path's operation sequence, while every value (weights, compositing, gradients)
+// stays FP64.  The records' FP32 predicate is the one the hit-set test checks bit for bit.
+struct orc_record {
+    const float* mu;       // (n,3)
+    const float* Mp;       // (n,9) rows of the bounding-ellipsoid transform
+    const int32_t* valid;  // (n)
+    const float* h;        // (n) ln|sigma/thr|
 };
 
 }  // extern "C"
@@ -528,11 +541,179 @@ void sh_basis_grad(double x, double y, double z, int deg, int exact, double* dX,
 
 // ---------------------------------------------------------------- per-ray march -------------
 
+// ---- FP32 evaluations of sigma*w, for measuring the rounding margin of the threshold test ----
+
+void quat_cols32(const float* q, float c0[3], float c1[3], float c2[3]) {
+    const float r = q[0], i = q[1], j = q[2], k = q[3];
+    c0[0] = 1.0f - 2.0f * (j * j + k * k), c0[1] = 2.f * (i * j + r * k), c0[2] = 2.f * (i * k - r * j);
+    c1[0] = 2.f * (i * j - r * k), c1[1] = 1.0f - 2.0f * (i * i + k * k), c1[2] = 2.f * (j * k + r * i);
+    c2[0] = 2.f * (i * k + r * j), c2[1] = 2.f * (j * k - r * i), c2[2] = 1.0f - 2.0f * (i * i + j * j);
+}
+
+// sigma*w as the reference evaluates it in FP32 (computeBufferForward,
+// forward/gaussians_aabb.cu:437-498): M = (R S^-1)^T from the float quaternion and scales,
+// x = o + d t, power = -0.5 |M (x - mu)|^2, w = expf(power).
+float sigw_ref32(const orc_scene& S, int64_t g, const Ray32& r, float t) {
+    float c0[3], c1[3], c2[3];
+    quat_cols32(S.quat + 4 * g, c0, c1, c2);
+    const float* s = S.scale + 3 * g;
+    const float is0 = 1.0f / s[0], is1 = 1.0f / s[1], is2 = 1.0f / s[2];
+    float U[3], V[3], W[3];
+    for (int k = 0; k < 3; ++k) {
+        U[k] = c0[k] * is0;
+        V[k] = c1[k] * is1;
+        W[k] = c2[k] * is2;
+    }
+    float X[3];
+    for (int k = 0; k < 3; ++k) {
+        const float x = r.o[k] + r.d[k] * t;
+        X[k] = x - S.pos[3 * g + k];
+    }
+    const float m0 = (X[0] * U[0] + X[1] * U[1]) + X[2] * U[2];
+    const float m1 = (X[0] * V[0] + X[1] * V[1]) + X[2] * V[2];
+    const float m2 = (X[0] * W[0] + X[1] * W[1]) + X[2] * W[2];
+    const float power = -0.5f * ((m0 * m0 + m1 * m1) + m2 * m2);
+    return S.dens[g] * std::exp(power);
+}
+
+// sigma*w as the HIP path evaluates it in FP32: the preprocessed record (rows of
+// M' = col_k(R) / (s_k delta), h = ln|sigma/thr|), the predicate's terms at the closest approach
+// t_c re-centred at x0 = o + t_c d, and w = exp2(-log2(e) (A + Cq (t - t_c)^2)).  Returns a
+// negative value for a Gaussian without a bounding ellipsoid (never sampled there).
+float sigw_closed32(const orc_scene& S, int64_t g, float thr, const Ray32& r, float t) {
+    const float sigma = S.dens[g];
+    const float ratio = sigma / thr;
+    const float l2 = std::log(ratio * ratio);
+    if (!(l2 > 0.0f && l2 < INFINITY)) return -1.0f;
+    const float delta = std::sqrt(l2), h = 0.5f * l2;
+    float c0[3], c1[3], c2[3];
+    quat_cols32(S.quat + 4 * g, c0, c1, c2);
+    const float* s = S.scale + 3 * g;
+    const float i0 = 1.0f / (s[0] * delta), i1 = 1.0f / (s[1] * delta), i2 = 1.0f / (s[2] * delta);
+    float M[3][3];
+    for (int k = 0; k < 3; ++k) {
+        M[0][k] = c0[k] * i0;
+        M[1][k] = c1[k] * i1;
+        M[2][k] = c2[k] * i2;
+    }
+    const float* mu = S.pos + 3 * g;
+    const float Ox = r.o[0] - mu[0], Oy = r.o[1] - mu[1], Oz = r.o[2] - mu[2];
+    float Op[3], Dp[3];
+    for (int k = 0; k < 3; ++k) {
+        Op[k] = (M[k][0] * Ox + M[k][1] * Oy) + M[k][2] * Oz;
+        Dp[k] = (M[k][0] * r.d[0] + M[k][1] * r.d[1]) + M[k][2] * r.d[2];
+    }
+    const float ddp = (Dp[0] * Dp[0] + Dp[1] * Dp[1]) + Dp[2] * Dp[2];
+    const float l = 1.0f / std::sqrt(ddp);
+    float Dh[3];
+    for (int k = 0; k < 3; ++k) Dh[k] = Dp[k] * l;
+    const float b = -((Op[0] * Dh[0] + Op[1] * Dh[1]) + Op[2] * Dh[2]);
+    const float t0 = b * l;
+    const float Cq = h * ddp;
+    float X0[3], O[3];
+    for (int k = 0; k < 3; ++k) X0[k] = (r.o[k] + t0 * r.d[k]) - mu[k];
+    for (int k = 0; k < 3; ++k) O[k] = M[k][0] * X0[0] + M[k][1] * X0[1] + M[k][2] * X0[2];
+    const float b2 = -(O[0] * Dh[0] + O[1] * Dh[1] + O[2] * Dh[2]);
+    float P[3];
+    for (int k = 0; k < 3; ++k) P[k] = O[k] + b2 * Dh[k];
+    const float tc = t0 + b2 * l;
+    const float A = h * (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+    constexpr float kLog2e = 1.4426950408889634f;
+    const float a = -A * kLog2e, c = -Cq * kLog2e;
+    const float tau = t - tc;
+    return sigma * std::exp2(std::fma(c, tau * tau, a));
+}
+
+// FP32 bounding-ellipsoid predicate on a record, operation for operation as the HIP path:
+// returns whether disc > 1e-7, with the interval [r1, r2].
+bool pred32(const float* mu, const float* m, const Ray32& r, float* r1, float* r2) {
+    const float Ox = r.o[0] - mu[0], Oy = r.o[1] - mu[1], Oz = r.o[2] - mu[2];
+    float Op[3], Dp[3];
+    for (int k = 0; k < 3; ++k) {
+        const float a0 = m[3 * k + 0] * Ox;
+        const float a1 = m[3 * k + 1] * Oy;
+        const float a2 = m[3 * k + 2] * Oz;
+        Op[k] = (a0 + a1) + a2;
+        const float b0 = m[3 * k + 0] * r.d[0];
+        const float b1 = m[3 * k + 1] * r.d[1];
+        const float b2 = m[3 * k + 2] * r.d[2];
+        Dp[k] = (b0 + b1) + b2;
+    }
+    const float ddp = (Dp[0] * Dp[0] + Dp[1] * Dp[1]) + Dp[2] * Dp[2];
+    const float l = 1.0f / std::sqrt(ddp);
+    float Dh[3];
+    for (int k = 0; k < 3; ++k) Dh[k] = Dp[k] * l;
+    const float b = -((Op[0] * Dh[0] + Op[1] * Dh[1]) + Op[2] * Dh[2]);
+    const float c = ((Op[0] * Op[0] + Op[1] * Op[1]) + Op[2] * Op[2]) - 1.0f;
+    float P[3];
+    for (int k = 0; k < 3; ++k) {
+        const float e = b * Dh[k];
+        P[k] = Op[k] + e;
+    }
+    const float pp = (P[0] * P[0] + P[1] * P[1]) + P[2] * P[2];
+    const float disc = 1.0f - pp;
+    if (!(disc > 1e-7f)) return false;
+    const float sd = std::sqrt(disc);
+    const float q = (b > 0.0f) ? b + sd : b - sd;
+    *r1 = (c / q) * l;
+    *r2 = q * l;
+    return true;
+}
+
+struct Dec32 {
+    orc_record rec;
+    const float* sigma;  // densities as given
+    float thr;
+};
+
+// sigma*w in FP32 from a record, as the HIP path samples a hit: the predicate's terms at the
+// closest approach re-centred at x0 = o + t_c d, and w = exp2(-log2(e) (A + Cq (t - t_c)^2)).
+float sigw32(const Dec32& D, int64_t g, const Ray32& r, float t) {
+    const float* mu = D.rec.mu + 3 * g;
+    const float* m = D.rec.Mp + 9 * g;
+    const float h = D.rec.h[g];
+    const float Ox = r.o[0] - mu[0], Oy = r.o[1] - mu[1], Oz = r.o[2] - mu[2];
+    float Op[3], Dp[3];
+    for (int k = 0; k < 3; ++k) {
+        Op[k] = (m[3 * k] * Ox + m[3 * k + 1] * Oy) + m[3 * k + 2] * Oz;
+        Dp[k] = (m[3 * k] * r.d[0] + m[3 * k + 1] * r.d[1]) + m[3 * k + 2] * r.d[2];
+    }
+    const float ddp = (Dp[0] * Dp[0] + Dp[1] * Dp[1]) + Dp[2] * Dp[2];
+    const float l = 1.0f / std::sqrt(ddp);
+    float Dh[3];
+    for (int k = 0; k < 3; ++k) Dh[k] = Dp[k] * l;
+    const float b = -((Op[0] * Dh[0] + Op[1] * Dh[1]) + Op[2] * Dh[2]);
+    const float t0 = b * l;
+    const float Cq = h * ddp;
+    float X0[3], O[3];
+    for (int k = 0; k < 3; ++k) X0[k] = (r.o[k] + t0 * r.d[k]) - mu[k];
+    for (int k = 0; k < 3; ++k) O[k] = (m[3 * k] * X0[0] + m[3 * k + 1] * X0[1]) + m[3 * k + 2] * X0[2];
+    const float b2 = -((O[0] * Dh[0] + O[1] * Dh[1]) + O[2] * Dh[2]);
+    float P[3];
+    for (int k = 0; k < 3; ++k) P[k] = O[k] + b2 * Dh[k];
+    const float tc = t0 + b2 * l;
+    const float A = h * ((P[0] * P[0] + P[1] * P[1]) + P[2] * P[2]);
+    constexpr float kLog2e = 1.4426950408889634f;
+    const float a = -A * kLog2e, c = -Cq * kLog2e;
+    const float tau = t - tc;
+    return D.sigma[g] * std::exp2(std::fma(c, tau * tau, a));
+}
+
+// Decision-matched mode: a sample decision within this relative distance of the threshold may be
+// taken differently by the GPU's native exp2 (about one ulp from the host's).
+const double kEpsDec = 1e-6;
+
+// Fragility kinds (RayOut.fragile bits): near-threshold sample decisions, a transmittance check
+// within rounding of T_eps, the origin-inside branch decided by a b within rounding of zero.
+enum { kFragSample = 1, kFragExit = 2, kFragQuirk = 4 };
+
 struct RayOut {
     double C[3];
     double depth;
     double T;
-    int fragile;
+    int fragile;      // kFrag* bits
+    int64_t nfrag;    // sample decisions (hit or near Gaussians) with |sigma*w/thr - 1| <= eps_w
+    double cmax;      // largest |colour component| among the ray's hit Gaussians and the background
     int64_t K;        // slab interactions (sum over slabs of |hit set|)
     int64_t P;        // distinct (ray, Gaussian) pairs among hit sets
     int64_t maxhits;  // largest per-slab hit set
@@ -551,9 +732,17 @@ inline bool near_rel(double a, double b, double rel) {
 // that Gaussian in that slab, and every such sample then has sigma*w at the threshold.  So the
 // oracle also evaluates, without accumulating, the samples of "near" Gaussians -- predicate
 // just failing (disc within kEpsD below 1e-7) or interval just missing the slab (within
-// kEpsT) -- and flags the ray when any sample of a hit or near Gaussian has
-// |sigma*w/thr - 1| <= kEpsW.
-const double kEpsW = 1e-4;  // relative, on sigma*w
+// kEpsT) -- and counts the samples of hit or near Gaussians with |sigma*w/thr - 1| <= eps_w.
+// eps_w is set from a measurement: orc_sigmaw_fp32_error() reports the largest relative FP32
+// error of sigma*w within 1 % of the threshold, for the reference's FP32 evaluation
+// (computeBufferForward, forward/gaussians_aabb.cu:437-498) and for the HIP path's closed form,
+// and the tests assert that eps_w is at least twice that error.
+//
+// Each counted decision bounds what a flip can do to the colour: the sample's density changes by
+// d <= thr (1 + eps_w), which moves C by at most d dt (3 cmax + max(cmax, bg)) (the sample's
+// opacity, its mixed colour and the transmittance of everything behind it), i.e. at most
+// 4 thr (1 + eps_w) dt max(cmax, 1) per decision.
+const double kEpsW = 1e-4;  // default relative margin on sigma*w
 const double kEpsD = 5e-5;  // absolute, on disc
 inline double eps_t(double t) { return 1e-6 * std::max(1.0, std::fabs(t)); }
 
@@ -567,12 +756,13 @@ struct Hit {
 // misses), sorted by (r1, g).
 void gather(const CpuBVH& B, const std::vector<Geo>& geo, const double o[3], const double d[3], double t0,
             double t1, std::vector<Hit>& hits, std::vector<int64_t>& scratch, int* fragile) {
+    // fragile: kFragQuirk is or-ed in
     hits.clear();
     bvh_query(B, o, d, t0 - eps_t(t0), t1 + eps_t(t1), scratch);
     for (int64_t g : scratch) {
         const Interval I = intersect64(geo[g], o, d);
         // the b > 0 branch decides hit / no hit when the origin is inside the ellipsoid
-        if (I.c < 0.0 && std::fabs(I.b) < 1e-5) *fragile = 1;
+        if (I.c < 0.0 && std::fabs(I.b) < 1e-5) *fragile |= kFragQuirk;
         Hit h{g, I.r1, I.r2, false};
         if (!I.valid) {
             if (!(geo[g].valid && I.disc > 1e-7 - kEpsD && I.b > 0.0)) continue;
@@ -647,8 +837,10 @@ inline double gauss_w(const Geo& G, const double x[3], double y[3]) {
 // computeBufferForward (forward/gaussians_aabb.cu:437-498) for one slab, FP64.
 void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const double* rgb,
                      const std::vector<Hit>& H, const std::vector<int>& cur, const Ray32& r, const SlabSamples& ts,
-                     SampleBuf& buf, int* fragile, int64_t* nc = nullptr, const std::vector<int>* near = nullptr) {
+                     SampleBuf& buf, int64_t* nfrag, int64_t* nc = nullptr, const std::vector<int>* near = nullptr,
+                     const Dec32* dec = nullptr) {
     const double thr = p.sigma_thr;
+    const double epsw = p.eps_w > 0.0f ? (double)p.eps_w : kEpsW;
     if (near)
         for (int a : *near) {
             const Geo& G = geo[H[a].g];
@@ -656,7 +848,7 @@ void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const dou
                 const double t = ts.t[i];
                 double x[3], y[3];
                 for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
-                if (G.sigma * gauss_w(G, x, y) >= thr * (1.0 - kEpsW)) *fragile = 1;
+                if (G.sigma * gauss_w(G, x, y) >= thr * (1.0 - epsw)) ++*nfrag;
             }
         }
     for (int a : cur) {
@@ -667,8 +859,16 @@ void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const dou
             for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
             const double w = gauss_w(G, x, y);
             const double sw = G.sigma * w;
-            if (std::fabs(sw / thr - 1.0) <= kEpsW) *fragile = 1;
-            if (sw > thr) {
+            bool take;
+            if (dec) {  // the HIP path's FP32 decision, the FP64 value
+                const float v = sigw32(*dec, H[a].g, r, ts.t[i]);
+                if (std::fabs((double)v / (double)dec->thr - 1.0) <= kEpsDec) ++*nfrag;
+                take = v > dec->thr;
+            } else {
+                if (std::fabs(sw / thr - 1.0) <= epsw) ++*nfrag;
+                take = sw > thr;
+            }
+            if (take) {
                 if (nc) ++*nc;
                 buf.sig[i] += sw;
                 buf.c0[i] += sw * rgb[3 * H[a].g + 0];
@@ -698,9 +898,28 @@ RayCtx make_ctx(const orc_params& p, const orc_camera& cam, int64_t pixel, const
     return c;
 }
 
+// Slab hit set of the decision-matched mode: the gathered hit and near entries that the FP32
+// record predicate accepts on [tmin, tmax] (the near entries cover every FP32-only hit: the FP32
+// predicate differs from the FP64 one far inside the near margins), ascending by Gaussian.
+const std::vector<int>& dec_members(const Dec32& D, const std::vector<Hit>& H, const SlabWalker& W, const Ray32& r,
+                                    float tmin, float tmax, std::vector<int>& out) {
+    out.clear();
+    for (const std::vector<int>* L : {&W.cur, &W.near})
+        for (int a : *L) {
+            const int64_t g = H[a].g;
+            float r1, r2;
+            if (D.rec.valid[g] && pred32(D.rec.mu + 3 * g, D.rec.Mp + 9 * g, r, &r1, &r2) &&
+                std::fmax(tmin, r1) <= std::fmin(tmax, r2))
+                out.push_back(a);
+        }
+    std::sort(out.begin(), out.end(), [&](int a, int b) { return H[a].g < H[b].g; });
+    return out;
+}
+
 // __raygen__rg forward (forward/gaussians_aabb.cu:521-612) + colorBlending (:500-519), FP64.
 RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const CpuBVH& B, const double* rgb,
-                     const RayCtx& c, std::vector<Hit>& hits, std::vector<int64_t>& scratch) {
+                     const RayCtx& c, std::vector<Hit>& hits, std::vector<int64_t>& scratch,
+                     const Dec32* dec = nullptr) {
     RayOut out;
     std::memset(&out, 0, sizeof(out));
     const double bg = p.white_bg ? 1.0 : 0.0;
@@ -711,6 +930,10 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
         return out;
     }
     gather(B, geo, c.o, c.d, c.tenter, c.texit, hits, scratch, &out.fragile);
+    out.cmax = bg;
+    for (const Hit& h : hits)
+        if (!h.near)
+            for (int k = 0; k < 3; ++k) out.cmax = std::max(out.cmax, std::fabs(rgb[3 * h.g + k]));
     SlabWalker W;
     W.hits = &hits;
     SampleBuf buf;
@@ -720,29 +943,32 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
     float tb = c.tenter;
     uint32_t seed = c.seed;
     std::vector<char> seen(hits.size(), 0);
+    std::vector<int> memv;
     while (tb < c.texit && T > eps) {
-        if (near_rel(T, eps, 1e-3)) out.fragile = 1;
+        if (near_rel(T, eps, 1e-3)) out.fragile |= kFragExit;
         const float tmin = std::fmax(c.tenter, tb);
         const float tmax = std::fmin(c.texit, tb + slab);
         out.slabs++;
         if (tmax > c.tenter) {
             W.step(tmin, tmax, &out.fragile);
             const SlabSamples ts(p, tb, seed);
-            if (W.cur.empty() && !W.near.empty()) {
+            const std::vector<int>& cur = dec ? dec_members(*dec, hits, W, c.r, tmin, tmax, memv) : W.cur;
+            if (!dec && W.cur.empty() && !W.near.empty()) {
                 buf.reset(p.spb);
                 const std::vector<int> none;
-                accumulate_slab(p, geo, rgb, hits, none, c.r, ts, buf, &out.fragile, nullptr, &W.near);
+                accumulate_slab(p, geo, rgb, hits, none, c.r, ts, buf, &out.nfrag, nullptr, &W.near);
             }
-            if (!W.cur.empty()) {
-                out.K += (int64_t)W.cur.size();
-                out.maxhits = std::max<int64_t>(out.maxhits, (int64_t)W.cur.size());
-                for (int a : W.cur)
+            if (!cur.empty()) {
+                out.K += (int64_t)cur.size();
+                out.maxhits = std::max<int64_t>(out.maxhits, (int64_t)cur.size());
+                for (int a : cur)
                     if (!seen[a]) {
                         seen[a] = 1;
                         out.P++;
                     }
                 buf.reset(p.spb);
-                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, ts, buf, &out.fragile, &out.nc, &W.near);
+                accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &out.nfrag, &out.nc, dec ? nullptr : &W.near,
+                                dec);
                 for (int i = 0; i < p.spb; ++i) {
                     const double s = buf.sig[i];
                     const double alpha = 1.0 - std::exp(-s * dt);
@@ -766,6 +992,7 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
     }
     for (int k = 0; k < 3; ++k) out.C[k] += T * bg;
     out.T = T;
+    if (out.nfrag > 0) out.fragile |= kFragSample;
     return out;
 }
 
@@ -802,7 +1029,7 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
                 for (size_t i = 0; i < hits.size(); ++i) cur[i] = (int)i;
                 out.K += (int64_t)hits.size();
                 buf.reset(p.spb);
-                int dummy = 0;
+                int64_t dummy = 0;
                 const SlabSamples ts(p, tb, seed);
                 accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &dummy);
                 for (int i = 0; i < p.spb; ++i) {
@@ -834,7 +1061,7 @@ inline void atomic_add(double* p, double v) {
 void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuBVH& B, const double* rgb,
                     const RayCtx& c, const double* Cfwd, const double* dLdC, double* dpos, double* dscale,
                     double* dquat, double* ddens, double* drgb, const orc_scene& S, std::vector<Hit>& hits,
-                    std::vector<int64_t>& scratch) {
+                    std::vector<int64_t>& scratch, const Dec32* dec = nullptr) {
     if (!(c.tenter < c.texit)) return;
     int fragile = 0;
     gather(B, geo, c.o, c.d, c.tenter, c.texit, hits, scratch, &fragile);
@@ -850,16 +1077,19 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
     uint32_t seed = c.seed;
     const int spb = p.spb;
     std::vector<double> alpha(spb), chat(3 * spb);
+    std::vector<int> memv;
     while (tb < c.texit && T > (double)p.t_eps) {
         const float tmin = std::fmax(c.tenter, tb);
         const float tmax = std::fmin(c.texit, tb + slab);
         if (tmax > c.tenter) {
             W.step(tmin, tmax, &fragile);
-            if (!W.cur.empty()) {
+            const std::vector<int>& cur = dec ? dec_members(*dec, hits, W, c.r, tmin, tmax, memv) : W.cur;
+            if (!cur.empty()) {
                 buf.reset(spb);
                 const SlabSamples ts(p, tb, seed);
                 seed = after_slab(p, seed);
-                accumulate_slab(p, geo, rgb, hits, W.cur, c.r, ts, buf, &fragile);
+                int64_t nfrag = 0;
+                accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &nfrag, nullptr, nullptr, dec);
                 for (int i = 0; i < spb; ++i) {
                     const double s = buf.sig[i];
                     double ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
@@ -869,7 +1099,7 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
                     alpha[i] = 1.0 - std::exp(-s * dt);
                 }
                 double Tn = T, Cn[3] = {Crem[0], Crem[1], Crem[2]};
-                for (int a : W.cur) {
+                for (int a : cur) {
                     const int64_t g = hits[a].g;
                     const Geo& G = geo[g];
                     double Ta = T;
@@ -887,7 +1117,7 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
                         double x[3], y[3];
                         for (int k = 0; k < 3; ++k) x[k] = (double)c.r.o[k] + t * (double)c.r.d[k];
                         const double w = gauss_w(G, x, y);
-                        if (G.sigma * w > thr) {
+                        if (dec ? sigw32(*dec, g, c.r, ts.t[i]) > dec->thr : G.sigma * w > thr) {
                             double wn = w * G.sigma;
                             if (cd > 0.0) wn /= cd;
                             grad_color += wn * al * Ta;
@@ -1136,9 +1366,11 @@ void orc_intersect(const orc_scene* S, float sigma_thr, int64_t nrays, const dou
 int orc_render(const orc_params* p, const orc_scene* S, const double* rgb, const float* bmin, const float* bmax,
                const orc_camera* cam, int64_t nrays, const int32_t* pixels, double* out_rgb, double* out_depth,
                double* out_opacity, int32_t* fragile, int64_t* K, int64_t* P, int64_t* maxhits, int64_t* nc,
-               int nthreads) {
+               int64_t* nfrag, double* cmax, const orc_record* rec, int nthreads) {
     const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
     const CpuBVH B = bvh_build(geo);
+    Dec32 dec{};
+    if (rec) dec = Dec32{*rec, S->dens, p->sigma_thr};
 #pragma omp parallel num_threads(nthreads)
     {
         std::vector<Hit> hits;
@@ -1146,12 +1378,14 @@ int orc_render(const orc_params* p, const orc_scene* S, const double* rgb, const
 #pragma omp for schedule(dynamic, 16)
         for (int64_t r = 0; r < nrays; ++r) {
             const RayCtx c = make_ctx(*p, *cam, pixels ? pixels[r] : r, bmin, bmax);
-            const RayOut o = march_forward(*p, geo, B, rgb, c, hits, scratch);
+            const RayOut o = march_forward(*p, geo, B, rgb, c, hits, scratch, rec ? &dec : nullptr);
             if (out_rgb)
                 for (int k = 0; k < 3; ++k) out_rgb[3 * r + k] = o.C[k];
             if (out_depth) out_depth[r] = o.depth;
             if (out_opacity) out_opacity[r] = 1.0 - o.T;
             if (fragile) fragile[r] = o.fragile;
+            if (nfrag) nfrag[r] = o.nfrag;
+            if (cmax) cmax[r] = o.cmax;
             if (K) K[r] = o.K;
             if (P) P[r] = o.P;
             if (maxhits) maxhits[r] = o.maxhits;
@@ -1159,6 +1393,59 @@ int orc_render(const orc_params* p, const orc_scene* S, const double* rgb, const
         }
     }
     return 0;
+}
+
+// Largest relative FP32 error of sigma*w, against FP64, over the samples of the rays' hit
+// Gaussians whose FP64 sigma*w lies within `band` (relative) of the threshold, walking every slab
+// from tenter to texit: out[0] for the reference's FP32 evaluation, out[1] for the HIP path's
+// closed form, out[2] the number of such samples.
+void orc_sigmaw_fp32_error(const orc_params* p, const orc_scene* S, const float* bmin, const float* bmax,
+                           const orc_camera* cam, int64_t nrays, const int32_t* pixels, double band, double* out,
+                           int nthreads) {
+    const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
+    const CpuBVH B = bvh_build(geo);
+    const double thr = p->sigma_thr;
+    double e_ref = 0.0, e_closed = 0.0, count = 0.0;
+#pragma omp parallel num_threads(nthreads) reduction(max : e_ref, e_closed) reduction(+ : count)
+    {
+        std::vector<Hit> hits;
+        std::vector<int64_t> scratch;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t r = 0; r < nrays; ++r) {
+            const RayCtx c = make_ctx(*p, *cam, pixels ? pixels[r] : r, bmin, bmax);
+            if (!(c.tenter < c.texit)) continue;
+            int quirk = 0;
+            gather(B, geo, c.o, c.d, c.tenter, c.texit, hits, scratch, &quirk);
+            SlabWalker W;
+            W.hits = &hits;
+            const float slab = p->dt * (float)p->spb;
+            uint32_t seed = c.seed;
+            for (float tb = c.tenter; tb < c.texit; tb = tb + slab) {
+                const float tmin = std::fmax(c.tenter, tb);
+                const float tmax = std::fmin(c.texit, tb + slab);
+                W.step(tmin, tmax, &quirk);
+                if (W.cur.empty()) continue;
+                const SlabSamples ts(*p, tb, seed);
+                seed = after_slab(*p, seed);
+                for (int a : W.cur) {
+                    const int64_t g = hits[a].g;
+                    for (int i = 0; i < p->spb; ++i) {
+                        double x[3], y[3];
+                        for (int k = 0; k < 3; ++k) x[k] = (double)c.r.o[k] + (double)ts.t[i] * (double)c.r.d[k];
+                        const double sw = geo[g].sigma * gauss_w(geo[g], x, y);
+                        if (!(std::fabs(sw / thr - 1.0) < band)) continue;
+                        count += 1.0;
+                        e_ref = std::max(e_ref, std::fabs((double)sigw_ref32(*S, g, c.r, ts.t[i]) / sw - 1.0));
+                        const float scl = sigw_closed32(*S, g, p->sigma_thr, c.r, ts.t[i]);
+                        if (scl >= 0.0f) e_closed = std::max(e_closed, std::fabs((double)scl / sw - 1.0));
+                    }
+                }
+            }
+        }
+    }
+    out[0] = e_ref;
+    out[1] = e_closed;
+    out[2] = count;
 }
 
 // Literal per-slab restatement (no BVH, no interval reuse): O(slabs * n) per ray.
@@ -1184,9 +1471,11 @@ int orc_render_literal(const orc_params* p, const orc_scene* S, const double* rg
 int orc_backward(const orc_params* p, const orc_scene* S, const double* rgb, const float* bmin, const float* bmax,
                  const orc_camera* cam, int64_t nrays, const int32_t* pixels, const double* ray_colors,
                  const double* dLdC, double* dpos, double* dscale, double* dquat, double* ddens, double* drgb,
-                 int nthreads) {
+                 const orc_record* rec, int nthreads) {
     const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
     const CpuBVH B = bvh_build(geo);
+    Dec32 dec{};
+    if (rec) dec = Dec32{*rec, S->dens, p->sigma_thr};
 #pragma omp parallel num_threads(nthreads)
     {
         std::vector<Hit> hits;
@@ -1197,7 +1486,7 @@ int orc_backward(const orc_params* p, const orc_scene* S, const double* rgb, con
             if (dl[0] == 0.0 && dl[1] == 0.0 && dl[2] == 0.0) continue;
             const RayCtx c = make_ctx(*p, *cam, pixels ? pixels[r] : r, bmin, bmax);
             march_backward(*p, geo, B, rgb, c, ray_colors + 3 * r, dl, dpos, dscale, dquat, ddens, drgb, *S, hits,
-                           scratch);
+                           scratch, rec ? &dec : nullptr);
         }
     }
     return 0;

@@ -287,8 +287,20 @@ class RayCaster:
         mu = torch.empty((n, 3), dtype=torch.float32, device=dev)
         Mp = torch.empty((n, 9), dtype=torch.float32, device=dev)
         valid = torch.empty(n, dtype=torch.int32, device=dev)
-        _lib.check(self._L.rg_export_predicate(self._h, _ptr(mu), _ptr(Mp), _ptr(valid), _stream()), self._h)
+        _lib.check(self._L.rg_export_predicate(self._h, _ptr(mu), _ptr(Mp), _ptr(valid), None, _stream()), self._h)
         return mu, Mp, valid
+
+    def export_record(self) -> dict:
+        """The FP32 per-Gaussian records the kernels decide with (original order), as numpy
+        arrays: mu (n,3), Mp (n,9), valid (n), h (n) = ln|sigma / thr|."""
+        dev = self._inputs["positions"].device
+        n = self.n
+        mu = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        Mp = torch.empty((n, 9), dtype=torch.float32, device=dev)
+        valid = torch.empty(n, dtype=torch.int32, device=dev)
+        h = torch.empty(n, dtype=torch.float32, device=dev)
+        _lib.check(self._L.rg_export_predicate(self._h, _ptr(mu), _ptr(Mp), _ptr(valid), _ptr(h), _stream()), self._h)
+        return {"mu": mu.cpu().numpy(), "Mp": Mp.cpu().numpy(), "valid": valid.cpu().numpy(), "h": h.cpu().numpy()}
 
     def debug_hits(self, view, pixels, max_slabs, max_pairs):
         dev = self._inputs["positions"].device

@@ -599,10 +599,10 @@ int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
 
 int64_t rg_alloc_epoch(const rg_handle* h) { return h ? h->alloc_epoch : -1; }
 
-int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, void* stream) {
+int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, float* hlog, void* stream) {
     if (!h || !mu || !Mp || !valid) return fail(h, RG_ERR_ARG, "null argument");
     if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
-    HIPCHK(h, rg::launch_export_predicate((int)h->n, h->bvh.iperm, h->geo, mu, Mp, valid,
+    HIPCHK(h, rg::launch_export_predicate((int)h->n, h->bvh.iperm, h->geo, mu, Mp, valid, hlog,
                                           static_cast<hipStream_t>(stream)));
     return RG_OK;
 }

@@ -345,7 +345,7 @@ __global__ void k_zero(float4* p, int64_t count4) {
 }
 
 __global__ void k_export_pred(int n, const int* __restrict__ iperm, const float4* __restrict__ geo, float* mu,
-                              float* Mp, int* valid) {
+                              float* Mp, int* valid, float* hlog) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int s = iperm[g];
@@ -356,6 +356,7 @@ __global__ void k_export_pred(int n, const int* __restrict__ iperm, const float4
     const float m[9] = {g1.x, g1.y, g1.z, g2.x, g2.y, g2.z, g3.x, g3.y, g3.z};
     for (int k = 0; k < 9; ++k) Mp[9 * g + k] = m[k];
     valid[g] = g1.w > 0.0f ? 1 : 0;
+    if (hlog) hlog[g] = g1.w;
 }
 
 inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
@@ -414,9 +415,10 @@ hipError_t launch_zero(float* p, int64_t count, hipStream_t s) {
 }
 
 hipError_t launch_export_predicate(int n, const int* iperm, const float4* geo, float* mu, float* Mp, int* valid,
+                                   float* hlog,
                                    hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_export_pred, dim3(blocks(n, 256)), dim3(256), 0, s, n, iperm, geo, mu, Mp, valid);
+    hipLaunchKernelGGL(k_export_pred, dim3(blocks(n, 256)), dim3(256), 0, s, n, iperm, geo, mu, Mp, valid, hlog);
     return hipGetLastError();
 }
 

@@ -86,7 +86,7 @@ hipError_t launch_zero(float* p, int64_t count, hipStream_t s);
 hipError_t launch_set_colors(int n, const float* rgb, const int* iperm, float4* table, hipStream_t s);
 hipError_t launch_rgba8(int64_t n, const float* rgb, uchar4* out, hipStream_t s);
 hipError_t launch_export_predicate(int n, const int* iperm, const float4* geo, float* mu, float* Mp, int* valid,
-                                   hipStream_t s);
+                                   float* hlog, hipStream_t s);
 
 // ---- launchers (lbvh.hip) ---------------------------------------------------------------------
 struct BvhBuffers {

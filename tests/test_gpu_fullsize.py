@@ -11,7 +11,7 @@ from oracle import oracle as O
 from ray_gauss_amd import scenes
 from ray_gauss_amd.batch import make_ray_batch
 
-from gpu_helpers import gpu_box, max_rel, oracle_forward, prepared_caster
+from gpu_helpers import check_image, gpu_box, max_rel, prepared_caster
 
 pytestmark = pytest.mark.gpu
 
@@ -31,19 +31,16 @@ def test_bvh_and_crop_parity(big):
     px = scenes.crop_pixels(wl.cameras[v], 600, 380, 32, 32)
     pxt = torch.from_numpy(px)
     rgb, depth, opac = rc.forward(v, pxt, depth=True, opacity=True)
-    ref = oracle_forward(wl.scene, wl.params, cams[v], gpu_box(rc), px)
-    ok = ref["fragile"] == 0
-    diff = np.abs(rgb.cpu().numpy() - ref["rgb"]).max(axis=1)
-    assert diff[ok].max() <= 1e-4, diff[ok].max()
-    assert (diff > 1e-4).mean() <= 0.05
-    # gradients on the crop, fragile rays masked on both sides
+    tok = rc.last_token
+    ref, ok, rec = check_image(rc, wl.scene, wl.params, cams[v], rgb, px)
+    # gradients on the crop against the decision-matched oracle, only its flagged rays masked
     dL = np.random.default_rng(3).normal(0, 1, (len(px), 3)) * ok[:, None]
-    rc.backward(v, pxt, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    rc.backward(v, pxt, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), tok)
     assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(wl.params, wl.scene, cams[v]["eye"], box[0], box[1], cams[v],
-                         rgb.cpu().numpy().astype(np.float64), dL, px)
+                         rgb.cpu().numpy().astype(np.float64), dL, px, record=rec)
     for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis"):
         e = max_rel(g[k].cpu().numpy(), og[k])
         assert e <= 1e-3, (k, e)

@@ -1,9 +1,11 @@
 """Parity of the HIP ray caster with the CPU FP64 oracle on the same seeded float32 inputs.
 
-Tolerances: images max-abs <= 1e-4 over rays the oracle does not flag as fragile (a threshold
-test within rounding distance of its boundary); gradients <= 1e-3 relative (max-abs error over
-max-abs value, per tensor) with the fragile rays' dL/dC set to zero on both sides; hit sets and
-the FP32 ray grid bit-exact.
+Tolerances: images max-abs <= 1e-4 against the decision-matched oracle on every ray not flagged
+(a decision within a few ulp of its boundary or a transmittance check within rounding of T_eps),
+every ray within its explicit bound against both oracle modes (gpu_helpers.check_image);
+gradients <= 1e-3 relative (max-abs error over max-abs value, per tensor) against the
+decision-matched oracle with only the flagged rays' dL/dC set to zero; hit sets and the FP32 ray
+grid bit-exact.
 """
 
 import numpy as np
@@ -14,7 +16,7 @@ from oracle import oracle as O
 from ray_gauss_amd import scenes
 from ray_gauss_amd.caster import TuneConfig
 
-from gpu_helpers import gpu_box, max_rel, oracle_forward, prepared_caster, rel_err
+from gpu_helpers import check_image, gpu_box, max_rel, oracle_forward, prepared_caster, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -76,31 +78,27 @@ def test_hit_sets_bit_exact_small(small):
 def test_forward_parity_small(small):
     wl, cam, rc, _ = small
     rgb, depth, opac = rc.forward(0, depth=True, opacity=True)
-    torch.cuda.synchronize()
-    ref = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc))
-    ok = ref["fragile"] == 0
-    assert ok.mean() > 0.95
-    err = np.abs(rgb.cpu().numpy() - ref["rgb"])[ok].max()
-    assert err <= IMG_TOL, err
-    assert np.abs(opac.cpu().numpy() - ref["opacity"])[ok].max() <= IMG_TOL
-    assert np.abs(depth.cpu().numpy() - ref["depth"])[ok].max() <= 1e-3 * max(1.0, ref["depth"].max())
     st = rc.stats()
-    assert st["slab_hits"] == int(ref["K"].sum()) or abs(st["slab_hits"] - ref["K"].sum()) <= 0.01 * ref["K"].sum()
+    ref, ok, _ = check_image(rc, wl.scene, wl.params, cam, rgb, max_flagged=0.01)
+    assert np.abs(opac.cpu().numpy() - ref["opacity"])[ok].max() <= IMG_TOL
+    assert np.abs(depth.cpu().numpy() - ref["depth"])[ok].max() <= 1e-4 * max(1.0, ref["depth"].max())
+    # the same hit sets slab by slab: the slab-hit and pair counts agree exactly
+    assert st["slab_hits"] == int(ref["K"].sum()) and st["pairs"] == int(ref["P"].sum())
 
 
 def test_backward_parity_small(small):
     wl, cam, rc, _ = small
     rgb, _, _ = rc.forward(0)
-    ref = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc))
-    ok = (ref["fragile"] == 0).astype(np.float64)
+    tok = rc.last_token
+    ref, ok, rec = check_image(rc, wl.scene, wl.params, cam, rgb, max_flagged=0.01)
     target = np.random.default_rng(1).uniform(0, 1, (4096, 3))
     dL = 2.0 * (rgb.cpu().numpy().astype(np.float64) - target) * ok[:, None]
-    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), tok)
     assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(wl.params, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64),
-                         dL)
+                         dL, record=rec)
     for k in ("positions", "scales", "quats", "densities", "sh"):
         e = max_rel(g[k].cpu().numpy(), og[k])
         assert e <= GRAD_TOL, (k, e, rel_err(g[k].cpu().numpy(), og[k]))
@@ -154,18 +152,15 @@ def test_blobs_crop_parity(gpu_device, builder):
     assert rc.check_bvh() == 0
     px = torch.from_numpy(scenes.crop_pixels(wl.cameras[0], 384, 384, 32, 32))
     rgb, _, _ = rc.forward(0, px)
-    ref = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc), px.numpy())
-    ok = ref["fragile"] == 0
-    assert ok.mean() > 0.9
-    err = np.abs(rgb.cpu().numpy() - ref["rgb"])[ok].max()
-    assert err <= IMG_TOL, err
+    tok = rc.last_token
+    ref, ok, rec = check_image(rc, wl.scene, wl.params, cam, rgb, px.numpy(), max_flagged=0.02)
     dL = np.random.default_rng(2).normal(0, 1, (px.shape[0], 3)) * ok[:, None]
-    rc.backward(0, px, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    rc.backward(0, px, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), tok)
     assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
     og = O.full_backward(wl.params, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64),
-                         dL, px.numpy())
+                         dL, px.numpy(), record=rec)
     for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis"):
         e = max_rel(g[k].cpu().numpy(), og[k])
         assert e <= GRAD_TOL, (k, e)
@@ -186,20 +181,18 @@ def test_random_sampling_parity(gpu_device, jitter, rnd):
     o, d = O.rays_rng(cam, px.numpy(), jitter, 17)
     assert np.array_equal(rd.cpu().numpy().view(np.uint32), d.view(np.uint32))
     rgb, _, _ = rc.forward(0)
-    ref = oracle_forward(wl.scene, prm, cam, gpu_box(rc))
-    ok = ref["fragile"] == 0
-    assert ok.mean() > 0.9
-    err = np.abs(rgb.cpu().numpy() - ref["rgb"])[ok].max()
-    assert err <= IMG_TOL, err
+    tok = rc.last_token
+    ref, ok, rec = check_image(rc, wl.scene, prm, cam, rgb, max_flagged=0.01)
     base = oracle_forward(wl.scene, wl.params, cam, gpu_box(rc))
     assert np.abs(ref["rgb"] - base["rgb"]).max() > 1e-4  # the settings do change the image
     target = np.random.default_rng(3).uniform(0, 1, (4096, 3))
     dL = 2.0 * (rgb.cpu().numpy().astype(np.float64) - target) * ok[:, None]
-    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), rc.last_token)
+    rc.backward(0, None, rgb, torch.from_numpy(dL.astype(np.float32)).cuda(), tok)
     assert rc.stats()["dropped"] == 0
     g = rc.finish_grads()
     box = gpu_box(rc)
-    og = O.full_backward(prm, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64), dL)
+    og = O.full_backward(prm, wl.scene, cam["eye"], box[0], box[1], cam, rgb.cpu().numpy().astype(np.float64), dL,
+                         record=rec)
     for k in ("positions", "scales", "quats", "densities", "sh"):
         e = max_rel(g[k].cpu().numpy(), og[k])
         assert e <= GRAD_TOL, (k, e, rel_err(g[k].cpu().numpy(), og[k]))

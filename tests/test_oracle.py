@@ -541,3 +541,104 @@ def test_depth_single_gaussian_analytic_sequence():
     out, _ = _render_one(sc, RenderParams(dt=0.0005, t_eps=0.0), cam)
     assert out["opacity"][0] > 0
     assert abs(out["depth"][0] / out["opacity"][0] - 3.0) < 2e-3
+
+
+# ---- fragility: measured FP32 margin, explicit per-ray bounds, decision-matched mode ----------
+
+def cpu_record(scene, thr=0.1):
+    """FP32 per-Gaussian records computed on the host the way the GPU preprocessing forms them
+    (rows of M' = col_k(R) / (s_k delta), h = ln|sigma / thr|); the oracle's self-tests use these
+    where the GPU tests use the kernels' own exported records."""
+    f = np.float32
+    q = scene.quats.astype(f)
+    r, i, j, k = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    two, one = f(2), f(1)
+    c0 = np.stack([one - two * (j * j + k * k), two * (i * j + r * k), two * (i * k - r * j)], 1)
+    c1 = np.stack([two * (i * j - r * k), one - two * (i * i + k * k), two * (j * k + r * i)], 1)
+    c2 = np.stack([two * (i * k + r * j), two * (j * k - r * i), one - two * (i * i + j * j)], 1)
+    ratio = scene.densities.astype(f) / f(thr)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        l2 = np.log(ratio * ratio).astype(f)
+        valid = (l2 > 0) & np.isfinite(l2)
+        delta = np.where(valid, np.sqrt(np.where(valid, l2, f(1))), f(0)).astype(f)
+        isp = np.where(valid[:, None], one / (scene.scales.astype(f) * delta[:, None]), f(0)).astype(f)
+    Mp = np.concatenate([c0 * isp[:, :1], c1 * isp[:, 1:2], c2 * isp[:, 2:3]], 1).astype(f)
+    h = np.where(valid, f(0.5) * l2, f(0)).astype(f)
+    return {"mu": scene.positions.astype(f), "Mp": Mp, "valid": valid.astype(np.int32), "h": h}
+
+
+def _crop_case(name):
+    wl = scenes.workload(name)
+    v = {"small4k": 0, "blobs100k": 0, "unbounded1m_8views": 3}[name]
+    cam = wl.cameras[v].launch_record()
+    if name == "small4k":
+        px = np.arange(0, 4096, 2, dtype=np.int32)
+    else:
+        x0, y0 = {"blobs100k": (384, 384), "unbounded1m_8views": (600, 380)}[name]
+        px = scenes.crop_pixels(wl.cameras[v], x0, y0, 32, 32)
+    return wl, cam, px
+
+
+@pytest.mark.parametrize("name", ["small4k", "blobs100k", "unbounded1m_8views"])
+def test_fp32_margin_covers_measured_error(name):
+    """The FP32 error of sigma*w within 1 % of the threshold, for the reference's own FP32
+    evaluation and for the HIP path's closed form, measured against FP64 on seeded crops: the
+    fragility margin EPS_W is at least twice the largest seen (the 3M-Gaussian workload measures
+    7.8e-5 and 1.5e-4, the largest of all)."""
+    wl, cam, px = _crop_case(name)
+    bmin, bmax = O.scene_box(wl.scene, wl.params.sigma_thr)
+    e = O.sigmaw_fp32_error(wl.params, wl.scene, bmin, bmax, cam, px)
+    assert e["samples"] > 1000
+    assert 0 < e["reference"] <= O.EPS_W / 2 and 0 < e["closed_form"] <= O.EPS_W / 2, e
+
+
+@pytest.mark.parametrize("name", ["small4k", "blobs100k"])
+def test_decision_matched_hit_sets_equal_fp32_brute_force(name):
+    """In the decision-matched mode each slab's hit set is the FP32 record predicate's: with no
+    early exit, the oracle's per-ray slab-hit count K equals the FP32 brute force over all
+    Gaussians (the near-miss candidates cover every FP32-only hit)."""
+    wl, cam, px = _crop_case(name)
+    prm = scenes.RenderParams(**{**wl.params.__dict__, "t_eps": 0.0})
+    rec = cpu_record(wl.scene, prm.sigma_thr)
+    bmin, bmax = O.scene_box(wl.scene, prm.sigma_thr)
+    rgb = O.sh_forward(wl.scene, cam["eye"])
+    out = O.render(prm, wl.scene, rgb, bmin, bmax, cam, px, record=rec)
+    o, d = O.rays(cam, px)
+    te, tx = O.ray_interval(cam, bmin, bmax, prm.t_near, prm.t_far, px)
+    _, counts = O.hits_fp32(rec["mu"], rec["Mp"], rec["valid"], o, d, te, tx, np.float32(prm.dt),
+                            prm.samples_per_slab, 100000, 1)
+    assert counts.sum() > 1000
+    assert np.array_equal(out["K"], counts)
+
+
+@pytest.mark.parametrize("name", ["blobs100k", "unbounded1m_8views"])
+def test_fp32_decisions_stay_within_the_fragile_bounds(name):
+    """An FP32-decision render (the oracle's decision-matched mode, deciding like the HIP path)
+    against the exact-decision render: rays with no decision within EPS_W of the threshold agree
+    to FP64 rounding, and EVERY ray lies within its explicit bound (1e-4 plus, per near-threshold
+    decision, 4 thr (1 + EPS_W) dt max(cmax, 1)); gradients with no row masked agree to 1e-3 of
+    each tensor's largest entry for the 100k-Gaussian crop and to 5e-3 for the 1M-Gaussian crop (its
+    FP32 decisions move the scale gradient by 1.6e-3: no FP32 implementation, the reference's
+    included, can meet 1e-3 against exact decisions there, which is why the GPU's gradients are
+    held to 1e-3 against the decision-matched mode)."""
+    wl, cam, px = _crop_case(name)
+    prm = wl.params
+    rec = cpu_record(wl.scene, prm.sigma_thr)
+    bmin, bmax = O.scene_box(wl.scene, prm.sigma_thr)
+    rgb = O.sh_forward(wl.scene, cam["eye"])
+    ex = O.render(prm, wl.scene, rgb, bmin, bmax, cam, px)
+    dm = O.render(prm, wl.scene, rgb, bmin, bmax, cam, px, record=rec)
+    err = np.abs(dm["rgb"] - ex["rgb"]).max(axis=1)
+    safe = ex["fragile"] == 0
+    assert err[safe].max() <= 1e-10
+    bound = O.fragile_bound(prm, ex)
+    assert np.isfinite(bound).all()
+    assert (err <= bound).all(), (err - bound).max()
+    if name == "unbounded1m_8views":
+        assert (err > 1e-10).sum() > 0  # the FP32 decisions do flip some of the fragile rays
+    dL = np.random.default_rng(4).normal(0, 1, (len(px), 3))
+    g_ex = O.full_backward(prm, wl.scene, cam["eye"], bmin, bmax, cam, ex["rgb"], dL, px)
+    g_dm = O.full_backward(prm, wl.scene, cam["eye"], bmin, bmax, cam, ex["rgb"], dL, px, record=rec)
+    for k in ("positions", "scales", "quats", "densities", "sh", "sg", "sharpness", "lobe_axis"):
+        e = np.abs(g_dm[k] - g_ex[k]).max() / np.abs(g_ex[k]).max()
+        assert e <= (1e-3 if name == "blobs100k" else 5e-3), (k, e)
