@@ -153,7 +153,9 @@ def full_step(args):
     R = int(rv.shape[0])
     target = torch.rand((R, 3), generator=torch.Generator(device=dev).manual_seed(0), device=dev)
     tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
-                      replay_words=args.replay_words, tree_builder=args.tree_builder)
+                      replay_words=args.replay_words, tree_builder=args.tree_builder,
+                      ray_chunk=args.ray_chunk, refill_lanes=args.refill_lanes,
+                      bwd_ray_chunk=args.bwd_ray_chunk)
     trainer = BatchedTrainer(RayCaster(wl.params, tune), buf, cams, rv, rp, target, loss_kind=0,
                              use_graph=not args.no_graph)
     opts = [FusedAdam([raw[k]], lr=FULL_STEP_LR[k]) for k in raw]
@@ -218,6 +220,12 @@ def main():
                     help="words per ray of the forward's hit-list records replayed by the backward (0: off)")
     ap.add_argument("--no-packet", action="store_true", help="per-lane LBVH traversal")
     ap.add_argument("--tree-builder", type=int, default=1, help="0: LBVH (Morton order), 1: binned SAH")
+    ap.add_argument("--ray-chunk", type=int, default=64,
+                    help="forward launch: 0 = one wavefront per 8x8 tile (grid launch); a multiple of 64 = "
+                         "persistent wavefronts taking that many rays at a time from per-XCD work queues")
+    ap.add_argument("--bwd-ray-chunk", type=int, default=0, help="backward launch, as --ray-chunk")
+    ap.add_argument("--refill-lanes", type=int, default=0,
+                    help="persistent forward: hand new rays to idle lanes once this many are idle (0: whole chunks)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
     ap.add_argument("--full-step", action="store_true",
@@ -273,7 +281,9 @@ def main():
     px_dev = [torch.from_numpy(p).to(dev) for p in pixels]
 
     tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
-                      replay_words=args.replay_words, tree_builder=args.tree_builder)
+                      replay_words=args.replay_words, tree_builder=args.tree_builder,
+                      ray_chunk=args.ray_chunk, refill_lanes=args.refill_lanes,
+                      bwd_ray_chunk=args.bwd_ray_chunk)
     rc = RayCaster(wl.params, tune)
 
     def prepare(rebuild=False):
@@ -378,7 +388,9 @@ def main():
                    "sh_degree": sc.sh_degree, "sg_lobes": L, "dt": wl.params.dt, "t_eps": wl.params.t_eps,
                    "hit_depth": tune.hit_depth, "seg_slabs": tune.seg_slabs,
                    "packet_traversal": tune.packet, "fwd_hit_depth": tune.fwd_hit_depth,
-                   "replay_words": tune.replay_words, "tree_builder": ("lbvh", "sah")[tune.tree_builder]},
+                   "replay_words": tune.replay_words, "tree_builder": ("lbvh", "sah")[tune.tree_builder],
+                   "ray_chunk": tune.ray_chunk, "refill_lanes": tune.refill_lanes,
+                   "bwd_ray_chunk": tune.bwd_ray_chunk},
         "tree_build_ms": build_ms,
         "per_view_rays_per_s": (R / dt_pv) if dt_pv else None,
         "batched_over_per_view": (dt_pv / dt_b) if dt_pv else None,

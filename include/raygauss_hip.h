@@ -66,6 +66,14 @@ typedef struct rg_config {
                                   ms at 1M), 1 binned SAH (tens of ms; fewer node visits per
                                   ray segment; reads back one counter per tree level, so not
                                   under stream capture)                                       */
+    int32_t ray_chunk;         /* launch structure of the forward kernel: 0 = one 8x8 pixel tile
+                                  per wavefront, grid launch; a multiple of 64 in 64..65536 =
+                                  persistent wavefronts (as many as the GPU holds) that take
+                                  ray_chunk rays at a time from per-XCD device work queues,
+                                  stealing from the other XCDs' queues when their own is empty */
+    int32_t refill_lanes;      /* persistent forward: 0 = a wavefront takes new rays when all its
+                                  lanes are done; 1..63 = as soon as that many lanes are idle   */
+    int32_t bwd_ray_chunk;     /* launch structure of the backward kernel, as ray_chunk         */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

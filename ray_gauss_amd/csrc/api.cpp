@@ -27,6 +27,7 @@ struct rg_handle {
     float* acc = nullptr;
     float* box = nullptr;
     unsigned long long* counters = nullptr;
+    unsigned long long* queue = nullptr;  // work queues of the persistent ray kernels (8 counters)
     rg::BvhBuffers bvh{};
     int64_t hist_cap = 0;
     // views
@@ -120,6 +121,11 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->replay_words < 0 || c->replay_words > (1 << 20) || (c->replay_words & 7))
         return fail(h, RG_ERR_ARG, "replay_words must be a multiple of 8 in 0..2^20");
     if (c->tree_builder != 0 && c->tree_builder != 1) return fail(h, RG_ERR_ARG, "tree_builder must be 0 or 1");
+    if (c->ray_chunk != 0 && (c->ray_chunk < 64 || c->ray_chunk > 65536 || (c->ray_chunk & 63)))
+        return fail(h, RG_ERR_ARG, "ray_chunk must be 0 or a multiple of 64 in 64..65536");
+    if (c->refill_lanes < 0 || c->refill_lanes > 64) return fail(h, RG_ERR_ARG, "refill_lanes must be in 0..64");
+    if (c->bwd_ray_chunk != 0 && (c->bwd_ray_chunk < 64 || c->bwd_ray_chunk > 65536 || (c->bwd_ray_chunk & 63)))
+        return fail(h, RG_ERR_ARG, "bwd_ray_chunk must be 0 or a multiple of 64 in 64..65536");
     return RG_OK;
 }
 
@@ -211,6 +217,9 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.rec_id = nullptr;
     p.rec_words = 0;
     p.rec_mode = 0;
+    p.ray_chunk = h->cfg.ray_chunk;
+    p.refill_lanes = h->cfg.refill_lanes;
+    p.queue = h->queue;
     return p;
 }
 
@@ -270,7 +279,9 @@ int rg_create(rg_handle** out, const rg_config* cfg) {
         return rc;
     }
     h->cfg = *cfg;
-    if ((rc = dalloc(h, &h->box, 8)) || (rc = dalloc(h, &h->counters, 16))) {
+    if ((rc = dalloc(h, &h->box, 8)) || (rc = dalloc(h, &h->counters, 16)) || (rc = dalloc(h, &h->queue, 16))) {
+        if (h->box) hipFree(h->box);
+        if (h->counters) hipFree(h->counters);
         delete h;
         return rc;
     }
@@ -284,7 +295,7 @@ int rg_create(rg_handle** out, const rg_config* cfg) {
 
 int rg_destroy(rg_handle* h) {
     if (!h) return RG_OK;
-    void* ptrs[] = {h->geo, h->brec, h->leafbox, h->acc, h->box, h->counters, h->bvh.keys[0], h->bvh.keys[1],
+    void* ptrs[] = {h->geo, h->brec, h->leafbox, h->acc, h->box, h->counters, h->queue, h->bvh.keys[0], h->bvh.keys[1],
                     h->bvh.vals[0], h->bvh.vals[1], h->bvh.hist, h->bvh.perm, h->bvh.iperm, h->bvh.nodes,
                     h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
                     h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb,
@@ -467,6 +478,7 @@ int rg_backward(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* pix
     const rg::DevGrad gr = dev_grad(h);
     rg::DevParams p = dev_params(h);
     rec_backward(h, job, replay_token, p);
+    p.ray_chunk = h->cfg.bwd_ray_chunk;
     HIPCHK(h, rg::launch_backward(dev_scene(h), p, job, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1, gr,
                                   dev_stats(h), s));
     return RG_OK;
@@ -515,6 +527,7 @@ int rg_backward_batch(rg_handle* h, const rg_ray_batch* b, const float* ray_colo
     const rg::DevGrad gr = dev_grad(h);
     rg::DevParams p = dev_params(h);
     rec_backward(h, j, replay_token, p);
+    p.ray_chunk = h->cfg.bwd_ray_chunk;
     HIPCHK(h, rg::launch_backward(dev_scene(h), p, j, h->cfg.hit_depth, ray_colors, dL_dC, nullptr, 0, 1, gr,
                                   dev_stats(h), s));
     return RG_OK;
@@ -544,6 +557,7 @@ int rg_train_batch(rg_handle* h, const rg_ray_batch* b, const float* target, int
     HIPCHK(h, rg::launch_forward(sc, p, j, fwd_depth(h), rgb, nullptr, nullptr, dev_stats(h), s));
     p = dev_params(h);
     rec_backward(h, j, token, p);
+    p.ray_chunk = h->cfg.bwd_ray_chunk;
     HIPCHK(h, rg::launch_backward(sc, p, j, h->cfg.hit_depth, rgb, nullptr, target, loss_kind, 0, dev_grad(h),
                                   dev_stats(h), s));
     if (out_loss)

@@ -58,6 +58,14 @@ struct DevParams {
     int2* rec_id;  // (view, pixel) of the ray whose records row r holds
     int rec_words;
     int rec_mode;
+    // launch structure of the ray kernels: ray_chunk 0 = one 8x8 tile per wavefront, grid-launched,
+    // tiles mapped to XCDs in contiguous bands; > 0 = persistent wavefronts taking ray_chunk rays
+    // at a time from per-XCD work queues (queue: 8 device counters, stealing from the other XCDs'
+    // queues when their own is empty); refill_lanes (forward, persistent): 0 = a wavefront takes new
+    // rays when all its lanes are done, k = as soon as k lanes are idle
+    int ray_chunk;
+    int refill_lanes;
+    unsigned long long* queue;
 };
 
 // Gradient accumulators (device-scope atomic adds; the finish kernel reads and clears them).

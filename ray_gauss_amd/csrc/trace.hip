@@ -73,11 +73,14 @@ __device__ __forceinline__ unsigned int slab_mask(const IList<CAP>& L, int cnt, 
 
 // Sampling quadratic of list entry e (as computed by intersect + refine_sampling at gather time).
 
-// drop: gradient records or replayed hits skipped by a bounds check (an internal error: every
-// parity test asserts zero); bad: rays whose view or pixel index is out of range (skipped).
+// Rays whose view or pixel index is out of range (skipped), and gradient records or replayed hits
+// skipped by a bounds check (an internal error: every parity test asserts none), go straight to
+// their device counters with an atomic on those rare paths, so they cost no registers.
 struct Counters {
-    unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0, rec_words = 0, rec_full = 0, drop = 0, bad = 0;
+    unsigned int K = 0, P = 0, ovf = 0, fallback = 0, gathers = 0, rec_words = 0, rec_full = 0;
 };
+
+__device__ __forceinline__ void count_drop(unsigned long long* drops) { atomicAdd(drops, 1ull); }
 
 // FP32 gradient add (device scope, relaxed: the result is read by a later kernel).
 __device__ __forceinline__ void gadd(float* p, float v) {
@@ -321,7 +324,7 @@ struct GradStage {
 // records were written by other lanes: the workgroup is one wavefront, and the barriers order
 // the LDS writes before these reads and these reads before the next records' writes.
 __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScene& sc, const DevGrad& gr,
-                                            int lane, unsigned int& dropped) {
+                                            int lane, unsigned long long* drops) {
     __syncthreads();
     const int j = lane & 15;
     for (int e0 = 0; e0 < cnt; e0 += 4) {
@@ -331,7 +334,7 @@ __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScen
             if ((unsigned)slot < (unsigned)sc.n && (unsigned)view < (unsigned)gr.num_views)
                 gadd(grad_addr(gr, sc.n, slot, view, j), G.val[e][j]);
             else if (j == 0)
-                ++dropped;  // never expected: counted so that tests see it
+                count_drop(drops);  // never expected: counted so that tests see it
         }
     }
     __syncthreads();
@@ -356,7 +359,7 @@ __device__ __forceinline__ void rs_step(const float* in, float* out, bool upper)
 // exchanges instead of 14 x 6).  Called by all 64 lanes together.
 __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int view, const DevScene& sc,
                                                   const DevGrad& gr, int lane, GradStage& G, int& nst,
-                                                  unsigned int& dropped) {
+                                                  unsigned long long* drops) {
     float a[16], b[8], c[4], d[2];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = k < kGradVals ? v[k] : 0.0f;
@@ -374,7 +377,7 @@ __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int 
         G.view[nst] = view;
     }
     if (++nst == kStage) {
-        stage_flush(G, kStage, sc, gr, lane, dropped);
+        stage_flush(G, kStage, sc, gr, lane, drops);
         nst = 0;
     }
 }
@@ -717,83 +720,99 @@ __device__ __forceinline__ bool tile_pixel(int64_t w, int width, int height, int
 
 __device__ __forceinline__ void flush_counters(const Counters& c, DevStats st) {
     // wave-level reduction, one atomic per counter per wave
-    unsigned int v[9] = {c.K, c.P, c.ovf, c.fallback, c.gathers, c.rec_words, c.rec_full, c.drop, c.bad};
-    const int idx[9] = {C_K, C_P, C_OVF, C_FALLBACK, C_GATHER, C_REC_WORDS, C_REC_FULL, C_DROP, C_BAD};
+    unsigned int v[7] = {c.K, c.P, c.ovf, c.fallback, c.gathers, c.rec_words, c.rec_full};
+    const int idx[7] = {C_K, C_P, C_OVF, C_FALLBACK, C_GATHER, C_REC_WORDS, C_REC_FULL};
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 7; ++k) {
         unsigned int x = v[k];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
         v[k] = x;
     }
     if (wave_lane() == 0) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
+        for (int k = 0; k < 7; ++k)
             if (v[k]) atomicAdd(&st.c[idx[k]], (unsigned long long)v[k]);
     }
 }
 
 // ------------------------------------------------------------------ per-ray bodies -----------
 
+// Starts a ray: clears its output and sets up its slab stepper; false when the ray has no slab
+// to process (it misses the scene box, or its first slab is below half an ulp of t).
+__device__ __forceinline__ bool forward_begin(const DevParams& p, const Ray& ray, RayOut& o, SegState& s) {
+    o.C[0] = o.C[1] = o.C[2] = 0.0f;
+    o.depth = 0.0f;
+    o.T = 1.0f;
+    if (!(ray.tenter < ray.texit)) return false;
+    if (!(add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter)) return false;
+    seg_init(s, ray, p);
+    return true;
+}
+
+// One slab of the forward (the reference's raygen loop body, forward/gaussians_aabb.cu:546-606):
+// the next slab with hits, its sample buffers, compositing.  Returns false when the ray is done.
+template <int SPB, int CAP, bool RNG, int RM>
+__device__ __forceinline__ bool forward_step(const DevScene& sc, const DevParams& p, const Ray& ray,
+                                             unsigned int& seed, const float4* __restrict__ rgbv, IList<CAP>& L,
+                                             int lane, Counters& ctr, RayOut& o, Rec& rc, SegState& s) {
+    if (!(o.T > p.t_eps)) return false;
+    const float first_tb = ray.tenter;
+    sync_gathers(s, ray, p);
+    float tmin, tmax;
+    if (!next_slab<CAP, RM>(s, sc, p, ray, L, lane, ctr, tmin, tmax, rc)) return false;
+    const float tb = s.tb;
+    float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
+#pragma unroll
+    for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
+    int nh = 0;
+    auto eval_hit = [&](int slot, const Isect& I, float sigma) {
+        accum_hit<SPB, RNG>(I, sigma, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
+        if (tb == first_tb || I.r1 > tmin) ctr.P++;
+        ++nh;
+    };
+    if (s.direct) {
+        int rstart = -1;
+        if constexpr (RM == 1) rstart = rec_direct_open(rc);
+        trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
+            float4 g0, g1, g2, g3;
+            load_rec(sc.geo, slot, g0, g1, g2, g3);
+            Isect I;
+            if (!intersect(g0, g1, g2, g3, ray, I)) return true;
+            if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
+            refine_sampling(g0, g1, g2, g3, ray, I);
+            if constexpr (RM == 1) rec_direct_push(rc, rstart, nh, slot);
+            eval_hit(slot, I, g0.w);
+            return true;
+        });
+        if constexpr (RM == 1) rec_direct_close(rc, rstart, nh, s.k);
+        s.dense = nh > CAP;
+    } else {
+        unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
+        while (m != 0u) {
+            const int e = __builtin_ctz(m);
+            m &= m - 1u;
+            const int slot = L.g[e][lane];
+            float sigma;
+            const Isect I = entry_hit(sc, ray, slot, L.r1[e][lane], L.r2[e][lane], sigma);
+            eval_hit(slot, I, sigma);
+        }
+    }
+    ctr.K += nh;
+    if (p.max_hits > 0 && nh > p.max_hits) ctr.ovf++;
+    if (nh > 0) {
+        composite<SPB, RNG>(sig, cr, cg, cb, tb, seed, p, o);
+        if (RNG && p.rnd_sample == 1) seed = skip_draws(seed, p.spb);  // the slab's draws are spent
+    }
+    return advance_slab(s, p);
+}
+
 template <int SPB, int CAP, bool RNG, int RM>
 __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& ray, unsigned int seed,
                             const float4* __restrict__ rgbv, IList<CAP>& L, int lane, Counters& ctr, RayOut& o,
                             Rec& rc) {
-    o.C[0] = o.C[1] = o.C[2] = 0.0f;
-    o.depth = 0.0f;
-    o.T = 1.0f;
-    if (!(ray.tenter < ray.texit)) return;
-    if (!(add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter)) return;
     SegState s;
-    seg_init(s, ray, p);
-    const float first_tb = ray.tenter;
-    while (o.T > p.t_eps) {
-        sync_gathers(s, ray, p);
-        float tmin, tmax;
-        if (!next_slab<CAP, RM>(s, sc, p, ray, L, lane, ctr, tmin, tmax, rc)) break;
-        const float tb = s.tb;
-        float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
-#pragma unroll
-        for (int i = 0; i < SPB; ++i) sig[i] = cr[i] = cg[i] = cb[i] = 0.0f;
-        int nh = 0;
-        auto eval_hit = [&](int slot, const Isect& I, float sigma) {
-            accum_hit<SPB, RNG>(I, sigma, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
-            if (tb == first_tb || I.r1 > tmin) ctr.P++;
-            ++nh;
-        };
-        if (s.direct) {
-            int rstart = -1;
-            if constexpr (RM == 1) rstart = rec_direct_open(rc);
-            trav(p, sc, ray, tmin, tmax, [&](int slot) -> bool {
-                float4 g0, g1, g2, g3;
-                load_rec(sc.geo, slot, g0, g1, g2, g3);
-                Isect I;
-                if (!intersect(g0, g1, g2, g3, ray, I)) return true;
-                if (!overlaps(tmin, tmax, I.r1, I.r2)) return true;
-                refine_sampling(g0, g1, g2, g3, ray, I);
-                if constexpr (RM == 1) rec_direct_push(rc, rstart, nh, slot);
-                eval_hit(slot, I, g0.w);
-                return true;
-            });
-            if constexpr (RM == 1) rec_direct_close(rc, rstart, nh, s.k);
-            s.dense = nh > CAP;
-        } else {
-            unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
-            while (m != 0u) {
-                const int e = __builtin_ctz(m);
-                m &= m - 1u;
-                const int slot = L.g[e][lane];
-                float sigma;
-                const Isect I = entry_hit(sc, ray, slot, L.r1[e][lane], L.r2[e][lane], sigma);
-                eval_hit(slot, I, sigma);
-            }
-        }
-        ctr.K += nh;
-        if (p.max_hits > 0 && nh > p.max_hits) ctr.ovf++;
-        if (nh > 0) {
-            composite<SPB, RNG>(sig, cr, cg, cb, tb, seed, p, o);
-            if (RNG && p.rnd_sample == 1) seed = skip_draws(seed, p.spb);  // the slab's draws are spent
-        }
-        if (!advance_slab(s, p)) break;
+    if (!forward_begin(p, ray, o, s)) return;
+    while (forward_step<SPB, CAP, RNG, RM>(sc, p, ray, seed, rgbv, L, lane, ctr, o, rc, s)) {
     }
 }
 
@@ -806,7 +825,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
                                     const float4* __restrict__ rgbv, float cur_tb, unsigned int cur_seed,
                                     float cur_tmin, float cur_tmax,
                                     const float* Aq, const float* Bq, const float* Eq, const float* dL, GradStage& G,
-                                    int& nst, int lane, const DevGrad& gr, int view, unsigned int& dropped) {
+                                    int& nst, int lane, const DevGrad& gr, int view, unsigned long long* drops) {
     const float ix = 1.0f / ray.dx, iy = 1.0f / ray.dy, iz = 1.0f / ray.dz;
     const int total = sc.total_nodes;
     bool live = dsc;
@@ -849,7 +868,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             float w[kGradVals];
 #pragma unroll
             for (int k = 0; k < kGradVals; ++k) w[k] = mine ? v[k] : 0.0f;
-            grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst, dropped);
+            grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst, drops);
             rem &= ~__ballot(mine);
         }
         idx = miss;
@@ -864,7 +883,8 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
 template <int SPB, int CAP, bool RNG, int RM>
 __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray, unsigned int seed,
                               const float4* __restrict__ rgbv, const float* Cfwd, const float* dL, IList<CAP>& L,
-                              GradStage& G, int lane, const DevGrad& gr, int view, Counters& ctr, Rec& rc) {
+                              GradStage& G, int lane, const DevGrad& gr, int view, Counters& ctr, Rec& rc,
+                              unsigned long long* drops) {
     int nst = 0;  // records in the staging area
     bool alive = active && (ray.tenter < ray.texit) && (add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter);
     float T = 1.0f;
@@ -922,7 +942,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     for (int i = 0; i < s.dcnt; ++i) {
                         const int slot = (int)*rec_at(rc, s.dpos + i);
                         if ((unsigned)slot < (unsigned)sc.n) direct_hit(slot);
-                        else ++ctr.drop;
+                        else count_drop(drops);
                     }
                     s.dense = nh > CAP;
                 } else if (s.direct) {
@@ -965,7 +985,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         // converged: slabs whose hit set overflowed the list scatter from a packet traversal
         if (__ballot(dsc) != 0ULL)
             direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, cg, dL, G,
-                                     nst, lane, gr, view, ctr.drop);
+                                     nst, lane, gr, view, drops);
         // wave-merged scatter over the prepared slabs' list hits
         int e = 0;
         while (true) {
@@ -977,8 +997,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                         my = key_base + sl;
                         break;
                     }
-                    ++ctr.drop;
-                    ++e;
+                    ++e;  // an out-of-range slot was counted when the slab was prepared
                 }
             } else if (has) {
                 while (e < cur_cnt) {
@@ -1015,16 +1034,14 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     refine_sampling(g0, g1, g2, g3, ray, I);
                     grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc,
                                         v);
-                } else {
-                    ++ctr.drop;
                 }
                 ++e;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
-            grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst, ctr.drop);
+            grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst, drops);
         }
     }
-    stage_flush(G, nst, sc, gr, lane, ctr.drop);
+    stage_flush(G, nst, sc, gr, lane, drops);
 }
 
 // ------------------------------------------------------------------ kernels ------------------
@@ -1064,6 +1081,43 @@ __device__ __forceinline__ int resolve(const RayJob& job, int64_t w, RayRef& q) 
     return RAY_OK;
 }
 
+// Writes a finished ray's colour (with the background behind its remaining transmittance),
+// depth and opacity, and closes its hit-list records.
+template <int RM>
+__device__ __forceinline__ void forward_finish(const DevParams& p, const RayRef& q, const RayOut& o, const Rec& rc,
+                                               Counters& ctr, float* __restrict__ out_rgb, float* __restrict__ out_depth,
+                                               float* __restrict__ out_opac) {
+    if constexpr (RM == 1) {
+        p.rec_len[q.out] = rc.pos | (rc.full ? (int)0x80000000u : 0);
+        p.rec_id[q.out] = make_int2(q.view, q.pixel);  // the backward replays only the same ray
+        ctr.rec_words += rc.pos;
+        ctr.rec_full += rc.full ? 1u : 0u;
+    }
+    out_rgb[3 * q.out + 0] = o.C[0] + o.T * p.bg;
+    out_rgb[3 * q.out + 1] = o.C[1] + o.T * p.bg;
+    out_rgb[3 * q.out + 2] = o.C[2] + o.T * p.bg;
+    if (out_depth) out_depth[q.out] = o.depth;
+    if (out_opac) out_opac[q.out] = 1.0f - o.T;
+}
+
+// A ray whose view or pixel index is out of range: NaN outputs, no records, counted.
+template <int RM>
+__device__ __forceinline__ void forward_bad(const DevParams& p, const RayRef& q, DevStats st,
+                                            float* __restrict__ out_rgb, float* __restrict__ out_depth,
+                                            float* __restrict__ out_opac) {
+    atomicAdd(&st.c[C_BAD], 1ull);
+    const float nan = __builtin_nanf("");
+    out_rgb[3 * q.out + 0] = nan;
+    out_rgb[3 * q.out + 1] = nan;
+    out_rgb[3 * q.out + 2] = nan;
+    if (out_depth) out_depth[q.out] = nan;
+    if (out_opac) out_opac[q.out] = nan;
+    if constexpr (RM == 1) {
+        p.rec_len[q.out] = 0;
+        p.rec_id[q.out] = make_int2(-1, -1);
+    }
+}
+
 // XCD-aware block order: consecutive work blocks land on the same XCD's L2.
 __device__ __forceinline__ int64_t block_item(int64_t b, int64_t nblocks) {
     const int64_t per = (nblocks + 7) >> 3;
@@ -1083,17 +1137,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB 
     RayRef q;
     const int rs = item < num_items ? resolve(job, item, q) : (int)RAY_NONE;
     if (rs == RAY_BAD) {
-        ++ctr.bad;
-        const float nan = __builtin_nanf("");
-        out_rgb[3 * q.out + 0] = nan;
-        out_rgb[3 * q.out + 1] = nan;
-        out_rgb[3 * q.out + 2] = nan;
-        if (out_depth) out_depth[q.out] = nan;
-        if (out_opac) out_opac[q.out] = nan;
-        if constexpr (RM == 1) {
-            p.rec_len[q.out] = 0;
-            p.rec_id[q.out] = make_int2(-1, -1);
-        }
+        forward_bad<RM>(p, q, st, out_rgb, out_depth, out_opac);
     } else if (rs == RAY_OK) {
         const rg_camera cam = job.cams[q.view];
         unsigned int seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
@@ -1105,19 +1149,69 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB 
             rc.end = p.rec_words;
         }
         forward_ray<SPB, CAP, RNG, RM>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o, rc);
-        if constexpr (RM == 1) {
-            p.rec_len[q.out] = rc.pos | (rc.full ? (int)0x80000000u : 0);
-            p.rec_id[q.out] = make_int2(q.view, q.pixel);  // the backward replays only the same ray
-            ctr.rec_words += rc.pos;
-            ctr.rec_full += rc.full ? 1u : 0u;
-        }
-        out_rgb[3 * q.out + 0] = o.C[0] + o.T * p.bg;
-        out_rgb[3 * q.out + 1] = o.C[1] + o.T * p.bg;
-        out_rgb[3 * q.out + 2] = o.C[2] + o.T * p.bg;
-        if (out_depth) out_depth[q.out] = o.depth;
-        if (out_opac) out_opac[q.out] = 1.0f - o.T;
+        forward_finish<RM>(p, q, o, rc, ctr, out_rgb, out_depth, out_opac);
     }
     flush_counters(ctr, st);
+}
+
+// One lane's ray of a backward pass.
+struct BwdRay {
+    bool active = false;
+    Ray ray = {};
+    unsigned int seed = 0u;
+    float dL[3] = {0.0f, 0.0f, 0.0f};
+    float Cf[3] = {0.0f, 0.0f, 0.0f};
+    int view = 0;
+    Rec rc;
+};
+
+// Sets up work item `item` of a backward: its rendered colour, dL/dC (given, or formed from the
+// per-pixel loss against target, whose value is added to loss), its hit-list records when they
+// were written for this same ray, and the ray itself.  A ray with zero dL/dC stays inactive.
+template <bool RNG, int RM>
+__device__ __forceinline__ void backward_setup(const DevScene& sc, const DevParams& p, const RayJob& job, int64_t item,
+                                               const float* __restrict__ ray_colors, const float* __restrict__ dLdC,
+                                               const float* __restrict__ target, int loss_kind, DevStats st,
+                                               BwdRay& b, float& loss) {
+    RayRef q;
+    const int rs = resolve(job, item, q);
+    if (rs == RAY_BAD) atomicAdd(&st.c[C_BAD], 1ull);
+    if (rs != RAY_OK) return;
+    b.Cf[0] = ray_colors[3 * q.out];
+    b.Cf[1] = ray_colors[3 * q.out + 1];
+    b.Cf[2] = ray_colors[3 * q.out + 2];
+    if (target) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float e = b.Cf[c] - target[3 * q.out + c];
+            if (loss_kind == 1) {
+                loss += fabsf(e);
+                b.dL[c] = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
+            } else {
+                loss += e * e;
+                b.dL[c] = 2.0f * e;
+            }
+        }
+    } else {
+        b.dL[0] = dLdC[3 * q.out];
+        b.dL[1] = dLdC[3 * q.out + 1];
+        b.dL[2] = dLdC[3 * q.out + 2];
+    }
+    if (b.dL[0] == 0.0f && b.dL[1] == 0.0f && b.dL[2] == 0.0f) return;
+    b.active = true;
+    b.view = q.view;
+    if constexpr (RM == 2) {
+        // records of this output row are replayed only when the forward wrote them for the same
+        // (view, pixel); otherwise the ray walks the tree
+        const int2 id = p.rec_id[q.out];
+        if (id.x == q.view && id.y == q.pixel) {
+            b.rc.w = rec_base(p.rec, q.out, p.rec_words);
+            b.rc.end = min(p.rec_len[q.out] & 0x7fffffff, p.rec_words);
+        }
+    }
+    const rg_camera cam = job.cams[q.view];
+    b.seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
+    b.ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &b.seed);
 }
 
 // Backward over a batch.  dL/dC comes from dLdC, or -- with target != nullptr -- from the
@@ -1135,61 +1229,216 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
     const int64_t nb = gridDim.x;
     const int64_t item = block_item(blockIdx.x, nb) * kWave + lane;
     Counters ctr;
-    RayRef q;
-    bool active = false;
-    Ray ray = {};
-    unsigned int seed = 0u;
-    float dL[3] = {0.0f, 0.0f, 0.0f}, Cf[3] = {0.0f, 0.0f, 0.0f};
+    BwdRay b;
     float loss = 0.0f;
-    int view = 0;
-    Rec rc;
-    const int rs = item < num_items ? resolve(job, item, q) : (int)RAY_NONE;
-    if (rs == RAY_BAD) ++ctr.bad;
-    if (rs == RAY_OK) {
-        Cf[0] = ray_colors[3 * q.out];
-        Cf[1] = ray_colors[3 * q.out + 1];
-        Cf[2] = ray_colors[3 * q.out + 2];
-        if (target) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float e = Cf[c] - target[3 * q.out + c];
-                if (loss_kind == 1) {
-                    loss += fabsf(e);
-                    dL[c] = e > 0.0f ? 1.0f : (e < 0.0f ? -1.0f : 0.0f);
-                } else {
-                    loss += e * e;
-                    dL[c] = 2.0f * e;
-                }
-            }
-        } else {
-            dL[0] = dLdC[3 * q.out];
-            dL[1] = dLdC[3 * q.out + 1];
-            dL[2] = dLdC[3 * q.out + 2];
-        }
-        if (dL[0] != 0.0f || dL[1] != 0.0f || dL[2] != 0.0f) {
-            active = true;
-            view = q.view;
-            if constexpr (RM == 2) {
-                // records of this output row are replayed only when the forward wrote them for
-                // the same (view, pixel); otherwise the ray walks the tree
-                const int2 id = p.rec_id[q.out];
-                if (id.x == q.view && id.y == q.pixel) {
-                    rc.w = rec_base(p.rec, q.out, p.rec_words);
-                    rc.end = min(p.rec_len[q.out] & 0x7fffffff, p.rec_words);
-                }
-            }
-            const rg_camera cam = job.cams[q.view];
-            seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
-            ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
-        }
-    }
-    backward_wave<SPB, CAP, RNG, RM>(sc, p, active, ray, seed, sc.rgb + (size_t)view * sc.n, Cf, dL, L, G, lane, gr, view,
-                            ctr, rc);
+    if (item < num_items) backward_setup<RNG, RM>(sc, p, job, item, ray_colors, dLdC, target, loss_kind, st, b, loss);
+    backward_wave<SPB, CAP, RNG, RM>(sc, p, b.active, b.ray, b.seed, sc.rgb + (size_t)b.view * sc.n, b.Cf, b.dL, L, G,
+                                     lane, gr, b.view, ctr, b.rc, st.c + C_DROP);
     if (count_stats) flush_counters(ctr, st);
     if (target) {
         const float ls = wave_sum64(loss);
         if (lane == 0 && ls != 0.0f) atomicAdd(reinterpret_cast<float*>(&st.c[C_LOSS]), ls);
     }
+}
+
+// ------------------------------------------------------------------ persistent kernels --------
+//
+// The same per-ray work as k_forward / k_backward, but launched as as many wavefronts as the GPU
+// holds at once, each taking p.ray_chunk work items (whole 8x8 tiles) at a time from a device work
+// queue until every queue is empty.  The items are cut into 8 contiguous bands, one per XCD (a
+// wavefront's XCD is its block index mod 8: blocks are dispatched round-robin over the XCDs), so
+// each XCD's L2 sees a compact part of the scene as with the grid launch; a wavefront whose band is
+// exhausted takes chunks from the other bands (work stealing), so no XCD idles while another still
+// has rays -- the grid launch's static bands (one view per XCD on the 8-view workload) leave the
+// XCDs with the cheaper views idle at the end.
+
+// Next chunk [lo, hi) of work items for a wavefront on XCD x: its own band's queue first, then the
+// other bands' in turn.  spent: bands this wavefront found exhausted.  Called by all active lanes
+// together; the results are wave-uniform.
+__device__ __forceinline__ bool take_chunk(const DevParams& p, int64_t num_items, int x, int64_t& lo, int64_t& hi,
+                                           unsigned int& spent) {
+    const int64_t tiles = (num_items + kWave - 1) / kWave;
+    const int64_t band = ((tiles + 7) / 8) * kWave;  // items per band, whole tiles
+    const bool leader = (int)threadIdx.x == __ffsll((long long)__ballot(1)) - 1;
+    for (int j = 0; j < 8; ++j) {
+        const int b = (x + j) & 7;
+        if (spent & (1u << b)) continue;
+        const int64_t start = (int64_t)b * band;
+        const int64_t len = min(band, num_items - start);
+        if (len <= 0) {
+            spent |= 1u << b;
+            continue;
+        }
+        unsigned long long got = 0ull;
+        if (leader) got = atomicAdd(&p.queue[b], (unsigned long long)p.ray_chunk);
+        const unsigned int glo = __builtin_amdgcn_readfirstlane((unsigned int)(got & 0xffffffffull));
+        const unsigned int ghi = __builtin_amdgcn_readfirstlane((unsigned int)(got >> 32));
+        const int64_t g = (int64_t)(((unsigned long long)ghi << 32) | glo);
+        if (g < len) {
+            lo = start + g;
+            hi = start + min(g + (int64_t)p.ray_chunk, len);
+            return true;
+        }
+        spent |= 1u << b;
+    }
+    return false;
+}
+
+// Lanes of mask below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+}
+
+// Persistent forward over whole tiles: each wavefront takes chunks from the queues and renders
+// them one tile (64 rays, one per lane) at a time, exactly as k_forward renders its tile.
+template <int SPB, int CAP, bool RNG, int RM>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB == 8 ? 4 : 3)))
+k_forward_pq(DevScene sc, DevParams p, RayJob job, int64_t num_items, float* __restrict__ out_rgb,
+             float* __restrict__ out_depth, float* __restrict__ out_opac, DevStats st) {
+    __shared__ IList<CAP> L;
+    const int lane = threadIdx.x;
+    int64_t lo = 0, hi = 0;
+    unsigned int spent = 0u;
+    while (take_chunk(p, num_items, blockIdx.x & 7, lo, hi, spent)) {
+        for (int64_t base = lo; base < hi; base += kWave) {
+            const int64_t item = base + lane;
+            Counters ctr;  // flushed per tile: no per-lane state lives across tiles
+            RayRef q;
+            const int rs = item < hi ? resolve(job, item, q) : (int)RAY_NONE;
+            if (rs == RAY_BAD) {
+                forward_bad<RM>(p, q, st, out_rgb, out_depth, out_opac);
+            } else if (rs == RAY_OK) {
+                const rg_camera cam = job.cams[q.view];
+                unsigned int seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
+                const Ray ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
+                RayOut o;
+                Rec rc;
+                if constexpr (RM == 1) {
+                    rc.w = rec_base(p.rec, q.out, p.rec_words);
+                    rc.end = p.rec_words;
+                }
+                forward_ray<SPB, CAP, RNG, RM>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o, rc);
+                forward_finish<RM>(p, q, o, rc, ctr, out_rgb, out_depth, out_opac);
+            }
+            flush_counters(ctr, st);
+        }
+    }
+}
+
+// Persistent forward that refills idle lanes (refill_lanes = k, 0 < k < 64): as soon as k lanes
+// are done they take the next rays of the chunk (and further chunks).  A lane's ray advances one
+// slab per wave step (forward_step), but a refilled lane starts at the scene box while the others
+// are deeper in, so the wavefront's lanes no longer query similar ray segments and its packet
+// traversals visit the union of two depth ranges: measured slower than whole tiles on the
+// 1M-Gaussian workload, kept as a tuning option.
+template <int SPB, int CAP, bool RNG, int RM>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(SPB == 8 ? 4 : 3)))
+k_forward_refill(DevScene sc, DevParams p, RayJob job, int64_t num_items, float* __restrict__ out_rgb,
+                 float* __restrict__ out_depth, float* __restrict__ out_opac, DevStats st) {
+    __shared__ IList<CAP> L;
+    const int lane = threadIdx.x;
+    const int xcd = blockIdx.x & 7;
+    const int refill = p.refill_lanes;
+    Counters ctr;
+    int64_t clo = 0, chi = 0;  // items of the current chunk not yet handed out (wave-uniform)
+    unsigned int spent = 0u;
+    bool more = true;          // the queues may hold more items (wave-uniform)
+    bool alive = false;        // this lane holds a ray that is not done
+    RayRef q;
+    q.view = 0;
+    q.pixel = 0;
+    q.out = 0;
+    q.active = false;
+    Ray ray = {};
+    unsigned int seed = 0u;
+    SegState s;
+    seg_init(s, ray, p);
+    RayOut o;
+    o.C[0] = o.C[1] = o.C[2] = 0.0f;
+    o.depth = 0.0f;
+    o.T = 1.0f;
+    Rec rc;
+    while (true) {
+        if (more && __popcll(__ballot(!alive)) >= refill) {
+            // hand the next items to the idle lanes, taking chunks from the queues as needed
+            bool need = !alive;
+            while (true) {
+                const unsigned long long nm = __ballot(need);
+                if (nm == 0ull) break;
+                if (clo >= chi && !take_chunk(p, num_items, xcd, clo, chi, spent)) {
+                    more = false;
+                    break;
+                }
+                const int64_t avail = chi - clo;
+                const int rank = lanes_below(nm);
+                if (need && rank < avail) {
+                    need = false;
+                    const int rs = resolve(job, clo + rank, q);
+                    if (rs == RAY_OK) {
+                        const rg_camera cam = job.cams[q.view];
+                        seed = RNG ? tea4(p.iteration, (unsigned int)q.pixel) : 0u;
+                        ray = make_ray(cam, q.pixel, sc.box, p.t_near, p.t_far, RNG ? p.jitter : 0, &seed);
+                        if constexpr (RM == 1) {
+                            rc.w = rec_base(p.rec, q.out, p.rec_words);
+                            rc.pos = 0;
+                            rc.end = p.rec_words;
+                            rc.full = false;
+                        }
+                        alive = forward_begin(p, ray, o, s);
+                        if (!alive) forward_finish<RM>(p, q, o, rc, ctr, out_rgb, out_depth, out_opac);
+                    } else if (rs == RAY_BAD) {
+                        forward_bad<RM>(p, q, st, out_rgb, out_depth, out_opac);
+                    }
+                }
+                clo += min((int64_t)__popcll(nm), avail);
+            }
+        }
+        if (__ballot(alive) == 0ull) {
+            if (!more) break;
+            continue;
+        }
+        if (alive && !forward_step<SPB, CAP, RNG, RM>(sc, p, ray, seed, sc.rgb + (size_t)q.view * sc.n, L, lane, ctr, o,
+                                                      rc, s)) {
+            forward_finish<RM>(p, q, o, rc, ctr, out_rgb, out_depth, out_opac);
+            alive = false;
+        }
+    }
+    flush_counters(ctr, st);
+}
+
+// Persistent backward: each wavefront takes chunks of whole tiles from the queues and runs the
+// depth-synchronised, wave-merged backward on one tile (64 rays) at a time.
+template <int SPB, int CAP, bool RNG, int RM>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3)))
+k_backward_pq(DevScene sc, DevParams p, RayJob job, int64_t num_items, const float* __restrict__ ray_colors,
+              const float* __restrict__ dLdC, const float* __restrict__ target, int loss_kind, int count_stats,
+              DevGrad gr, DevStats st) {
+    __shared__ IList<CAP> L;
+    __shared__ GradStage G;
+    const int lane = threadIdx.x;
+    const int xcd = blockIdx.x & 7;
+    int64_t lo = 0, hi = 0;
+    unsigned int spent = 0u;
+    while (take_chunk(p, num_items, xcd, lo, hi, spent)) {
+        for (int64_t base = lo; base < hi; base += kWave) {
+            const int64_t item = base + lane;
+            Counters ctr;  // flushed per tile: no per-lane state lives across tiles
+            float loss = 0.0f;
+            BwdRay b;
+            if (item < hi) backward_setup<RNG, RM>(sc, p, job, item, ray_colors, dLdC, target, loss_kind, st, b, loss);
+            backward_wave<SPB, CAP, RNG, RM>(sc, p, b.active, b.ray, b.seed, sc.rgb + (size_t)b.view * sc.n, b.Cf, b.dL,
+                                             L, G, lane, gr, b.view, ctr, b.rc, st.c + C_DROP);
+            if (count_stats) flush_counters(ctr, st);
+            if (target) {
+                const float ls = wave_sum64(loss);
+                if (lane == 0 && ls != 0.0f) atomicAdd(reinterpret_cast<float*>(&st.c[C_LOSS]), ls);
+            }
+        }
+    }
+}
+
+__global__ void k_reset_queue(unsigned long long* q) {
+    if (threadIdx.x < 8) q[threadIdx.x] = 0ull;
 }
 
 // Verification: per-slab hit sets from the same traversal + predicate (every slab to texit).
@@ -1302,13 +1551,63 @@ hipError_t launch_reset_counters(DevStats st, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Number of wavefronts a persistent kernel is launched with: as many as the GPU holds at once
+// (occupancy x CUs, a multiple of 8 so that every XCD gets the same number), and no more than
+// there are chunks of work.
+template <typename K>
+static int64_t persistent_grid(K kernel, int64_t chunks) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), kWave, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    const int64_t resident = (int64_t)cus * per_cu;
+    int64_t g = chunks < resident ? chunks : resident;
+    g = ((g + 7) / 8) * 8;
+    return g > 0 ? g : 8;
+}
+
+template <typename K, typename... A>
+static void launch_persistent(K kernel, int64_t items, const DevParams& p, hipStream_t s, A... args) {
+    const int64_t chunks = (items + p.ray_chunk - 1) / p.ray_chunk;
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, p.queue);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)persistent_grid(kernel, chunks)), dim3(kWave), 0, s, args...);
+}
+
+#define RG_PQ_CAP(SPBC, RNGC, RMC, CAPV, KERNEL, ...)                                                           \
+    do {                                                                                                             \
+        if (CAPV <= 8) launch_persistent(KERNEL<SPBC, 8, RNGC, RMC>, __VA_ARGS__);                                  \
+        else if (CAPV <= 16) launch_persistent(KERNEL<SPBC, 16, RNGC, RMC>, __VA_ARGS__);                           \
+        else launch_persistent(KERNEL<SPBC, 32, RNGC, RMC>, __VA_ARGS__);                                           \
+    } while (0)
+#define RG_PQ_CAP6(SPBC, RNGC, RMC, CAPV, KERNEL, ...)                                                          \
+    do {                                                                                                             \
+        if (CAPV <= 6) launch_persistent(KERNEL<SPBC, 6, RNGC, RMC>, __VA_ARGS__);                                  \
+        else if (CAPV <= 12) launch_persistent(KERNEL<SPBC, 12, RNGC, RMC>, __VA_ARGS__);                           \
+        else if (CAPV <= 16) launch_persistent(KERNEL<SPBC, 16, RNGC, RMC>, __VA_ARGS__);                           \
+        else launch_persistent(KERNEL<SPBC, 32, RNGC, RMC>, __VA_ARGS__);                                           \
+    } while (0)
+
 hipError_t launch_forward(const DevScene& sc, const DevParams& p, const RayJob& job, int hit_depth, float* out_rgb,
                           float* out_depth, float* out_opacity, DevStats st, hipStream_t s) {
     const int64_t items = job.num_rays;  // already the padded item count in implicit mode
     if (items <= 0) return hipSuccess;
+    const bool rng = p.jitter || p.rnd_sample;
+    if (p.ray_chunk > 0 && p.refill_lanes > 0 && p.refill_lanes < kWave) {
+        RG_DISPATCH_ANY(RG_PQ_CAP6, p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 1, 1, k_forward_refill, items, p,
+                        s, sc, p, job, items, out_rgb, out_depth, out_opacity, st);
+        return hipGetLastError();
+    }
+    if (p.ray_chunk > 0) {
+        RG_DISPATCH_ANY(RG_PQ_CAP6, p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 1, 1, k_forward_pq, items, p, s,
+                        sc, p, job, items, out_rgb, out_depth, out_opacity, st);
+        return hipGetLastError();
+    }
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
-    const bool rng = p.jitter || p.rnd_sample;
     RG_DISPATCH_FWD(p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 1, k_forward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
                 items, out_rgb, out_depth, out_opacity, st);
     return hipGetLastError();
@@ -1319,9 +1618,14 @@ hipError_t launch_backward(const DevScene& sc, const DevParams& p, const RayJob&
                            int count_stats, DevGrad gr, DevStats st, hipStream_t s) {
     const int64_t items = job.num_rays;
     if (items <= 0) return hipSuccess;
+    const bool rng = p.jitter || p.rnd_sample;
+    if (p.ray_chunk > 0) {
+        RG_DISPATCH_ANY(RG_PQ_CAP, p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 2, 2, k_backward_pq, items, p, s,
+                        sc, p, job, items, ray_colors, dLdC, target, loss_kind, count_stats, gr, st);
+        return hipGetLastError();
+    }
     const int64_t nb = (items + kWave - 1) / kWave;
     const int64_t nbp = ((nb + 7) / 8) * 8;
-    const bool rng = p.jitter || p.rnd_sample;
     RG_DISPATCH(p.spb <= 8 ? 8 : 16, hit_depth, rng, p.rec_mode == 2, k_backward, dim3((unsigned)nbp), dim3(kWave), 0, s, sc, p, job,
                 items, ray_colors, dLdC, target, loss_kind, count_stats, gr, st);
     return hipGetLastError();

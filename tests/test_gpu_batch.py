@@ -275,3 +275,52 @@ def test_dropin_jitter_and_random_samples(gpu_device):
     rc.set_views([wl.cameras[0].launch_record()])
     rgb, _, _ = rc.forward(0)
     assert torch.equal(rgb, imgs[1])
+
+
+@pytest.mark.parametrize("chunk,refill", [(64, 0), (128, 0), (256, 0), (64, 32), (64, 8)])
+def test_persistent_kernels_equal_grid_launch(blobs, gpu_device, chunk, refill):
+    """Persistent wavefronts taking `chunk` rays at a time from per-XCD work queues (with work
+    stealing), optionally refilling idle forward lanes: every ray is processed exactly once with
+    the same slabs and hit lists, so images are bit-identical to the grid launch and gradients
+    equal up to the order of atomic sums -- for a separate forward / backward (records replayed)
+    and for the fused train step."""
+    sc, prm, cams, pixels, rc0, t0, rv, rp, offs = blobs
+    gen = torch.Generator(device=gpu_device).manual_seed(7)
+    target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
+    res = {}
+    for ch, rf in ((0, 0), (chunk, refill)):
+        rc, _ = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(ray_chunk=ch, refill_lanes=rf, bwd_ray_chunk=ch))
+        rgb, depth, opac = rc.forward_batch(rv, rp, depth=True, opacity=True)
+        st = rc.stats()
+        rc.backward_batch(rv, rp, rgb, 2.0 * (rgb - target), rc.last_token)
+        assert rc.stats()["dropped"] == 0
+        g = {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}
+        out = torch.empty_like(rgb)
+        loss = torch.zeros(1, device=gpu_device)
+        rc.train_batch(rv, rp, target, 0, out, loss)
+        gt = {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}
+        res[ch] = (rgb, depth, opac, st, g, out, float(loss.item()), gt)
+    a, b = res[0], res[chunk]
+    for i in (0, 1, 2, 5):
+        assert torch.equal(a[i], b[i]), i
+    # the same slab hits and pairs; with whole tiles per wavefront also the same gathers and records
+    for k in ("slab_hits", "pairs") + (("gathers", "replay_words") if refill == 0 else ()):
+        assert a[3][k] == b[3][k], (k, a[3][k], b[3][k])
+    assert abs(a[6] - b[6]) <= 1e-6 * abs(a[6])
+    for k, v in a[4].items():
+        assert max_rel(b[4][k], v) <= 1e-5, k
+        assert max_rel(b[7][k], a[7][k]) <= 1e-5, k
+
+
+def test_persistent_full_view_and_pixel_lists(gpu_device):
+    """The persistent kernels over the implicit full-view enumeration (8x8 tiles with padding) and
+    over a pixel list give the grid launch's images."""
+    wl = scenes.workload("small4k")
+    cam = wl.cameras[0].launch_record()
+    px = torch.arange(3, 4096, 5, dtype=torch.int32)
+    out = {}
+    for ch in (0, 128):
+        rc, _ = prepared_caster(wl.scene, wl.params, [cam], gpu_device,
+                                TuneConfig(ray_chunk=ch, refill_lanes=16 if ch else 0, bwd_ray_chunk=ch))
+        out[ch] = (rc.forward(0)[0], rc.forward(0, px)[0])
+    assert torch.equal(out[0][0], out[128][0]) and torch.equal(out[0][1], out[128][1])
