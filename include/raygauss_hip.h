@@ -220,6 +220,13 @@ int rg_get_stats(rg_handle* h, rg_stats* out, void* stream);
  * unchanged.  No reference counterpart (the reference captures no graphs). */
 int64_t rg_alloc_epoch(const rg_handle* h);
 
+/* Device memory the handle holds: the sum of its live allocations (the tree, per-Gaussian
+ * records, colour tables, gradient accumulators, hit-list records), bytes; record_bytes (may be
+ * NULL) receives the share of the hit-list records that rg_backward replays (rays x replay_words
+ * x 4 B plus 12 B per ray).  -1 for a null handle.  No reference counterpart: the reference
+ * keeps no state between its forward and backward launches. */
+int64_t rg_device_bytes(const rg_handle* h, int64_t* record_bytes);
+
 /* Debug/verification: the FP32 per-Gaussian records in original order, exactly as the kernels use
  * them -- mu (n,3), Mp (n,9) rows of S'^-1 R^T, valid (n), and (optional, may be NULL) hlog (n) =
  * h = ln|sigma / thr|, the scale of the sampling exponent -- so that a checker can take the same

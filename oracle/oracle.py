@@ -354,3 +354,59 @@ def intersect(scene, o, d, sigma_thr=0.1):
     lib().orc_intersect(C.byref(h.s), C.c_float(sigma_thr), C.c_int64(nr), _p(o), _p(d), _p(out), _p(valid))
     return {"r1": out[..., 0], "r2": out[..., 1], "disc": out[..., 2], "b": out[..., 3], "c": out[..., 4],
             "valid": valid.astype(bool)}
+
+
+# ------------------------------------------------------------------ FP32 CPU baseline ---------
+# liboracle_rg32.so: the same source as the oracle built with real = float, without the oracle's
+# fragility bookkeeping, for x86-64-v3 (AVX2 + FMA).  A speed reference, not a checker.
+
+_LIB32_PATH = os.path.join(_HERE, "_build", "liboracle_rg32.so")
+_lib32 = None
+
+
+def cpu_model():
+    """(model name, has AVX2 and FMA) of this host from /proc/cpuinfo."""
+    name, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and name == "unknown":
+                    name = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return name, {"avx2", "fma"} <= flags
+
+
+def baseline_lib():
+    global _lib32
+    if _lib32 is None:
+        if not cpu_model()[1]:
+            raise RuntimeError("the FP32 CPU baseline is built for x86-64-v3; this host lacks AVX2/FMA")
+        if not os.path.exists(_LIB32_PATH):
+            build()
+        _lib32 = C.CDLL(_LIB32_PATH)
+        assert _lib32.orc_baseline_real_bytes() == 4
+    return _lib32
+
+
+def baseline_step(prm, scene, rgb, bmin, bmax, cam_rec, pixels, target, threads=None):
+    """One FP32 training pass of the ray caster over the given rays of one view (geometry + BVH,
+    forward, L2 gradient against `target`, backward).  Returns the colours, the gradients and the
+    seconds of the three phases."""
+    h = _SceneHolder(scene)
+    n = scene.n
+    px = np.ascontiguousarray(pixels, np.int32)
+    nr = len(px)
+    f = lambda *shape: np.zeros(shape, np.float32)  # noqa: E731
+    out = f(nr, 3)
+    g = {"positions": f(n, 3), "scales": f(n, 3), "quats": f(n, 4), "densities": f(n), "rgb": f(n, 3)}
+    times = np.zeros(3)
+    ps = params_struct(prm)
+    cs = camera_struct(cam_rec)
+    baseline_lib().orc_baseline_step(
+        C.byref(ps), C.byref(h.s), _p(_f32(rgb)), _p(_f32(bmin)), _p(_f32(bmax)), C.byref(cs), C.c_int64(nr), _p(px),
+        _p(_f32(target)), _p(out), _p(g["positions"]), _p(g["scales"]), _p(g["quats"]), _p(g["densities"]),
+        _p(g["rgb"]), _p(times), C.c_int(nthreads(threads)))
+    return {"rgb": out, "grads": g, "times": tuple(float(t) for t in times)}

@@ -47,6 +47,12 @@
 // OptiX-internal and cannot be reproduced, so no hits are dropped here (the per-slab maximum is
 // reported so tests can assert the 1024 limit is never reached).
 
+//
+// FP32 CPU baseline build (ORC_BASELINE, liboracle_rg32.so): the same source with `real` = float
+// and single-precision constants, minus the fragility bookkeeping only the oracle needs (near
+// misses, rounding margins, near-threshold counts).  It exports orc_baseline_step() alone and is
+// a speed reference for the same algorithm, not a checker.
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -54,6 +60,17 @@
 #include <cstring>
 #include <vector>
 #include <omp.h>
+
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+
+#ifdef ORC_BASELINE
+constexpr bool kOracle = false;
+#else
+constexpr bool kOracle = true;
+#endif
 
 extern "C" {
 
@@ -221,18 +238,18 @@ inline uint32_t after_slab(const orc_params& p, uint32_t seed) {
 // ---------------------------------------------------------------- FP64 geometry -------------
 
 struct Geo {
-    double mu[3];
-    double R[3][3];     // R[row][col]; columns are the rotated axes (quaternion_to_matrix)
-    double s[3];
-    double sigma;
-    double M[3][3];     // rows: col_k(R)/s_k       (unenlarged, sampling)
-    double Mp[3][3];    // rows: col_k(R)/(s_k*delta) (bounding ellipsoid, predicate)
+    real mu[3];
+    real R[3][3];     // R[row][col]; columns are the rotated axes (quaternion_to_matrix)
+    real s[3];
+    real sigma;
+    real M[3][3];     // rows: col_k(R)/s_k       (unenlarged, sampling)
+    real Mp[3][3];    // rows: col_k(R)/(s_k*delta) (bounding ellipsoid, predicate)
     bool valid;         // bounding ellipsoid exists: ln((sigma/thr)^2) > 0
 };
 
 // quaternion_to_matrix: q = (r,i,j,k) = (w,x,y,z), not normalised.
-void quat_to_R(const float* q, double R[3][3]) {
-    const double r = q[0], i = q[1], j = q[2], k = q[3];
+void quat_to_R(const float* q, real R[3][3]) {
+    const real r = q[0], i = q[1], j = q[2], k = q[3];
     // column 0
     R[0][0] = 1.0 - 2.0 * (j * j + k * k);
     R[1][0] = 2.0 * (i * j + r * k);
@@ -247,7 +264,7 @@ void quat_to_R(const float* q, double R[3][3]) {
     R[2][2] = 1.0 - 2.0 * (i * i + j * j);
 }
 
-Geo make_geo(const orc_scene& S, int64_t g, double thr) {
+Geo make_geo(const orc_scene& S, int64_t g, real thr) {
     Geo G;
     for (int k = 0; k < 3; ++k) {
         G.mu[k] = S.pos[3 * g + k];
@@ -255,10 +272,10 @@ Geo make_geo(const orc_scene& S, int64_t g, double thr) {
     }
     G.sigma = S.dens[g];
     quat_to_R(S.quat + 4 * g, G.R);
-    const double ratio = G.sigma / thr;
-    const double l2 = std::log(ratio * ratio);
+    const real ratio = G.sigma / thr;
+    const real l2 = std::log(ratio * ratio);
     G.valid = l2 > 0.0 && std::isfinite(l2);
-    const double delta = G.valid ? std::sqrt(l2) : 0.0;
+    const real delta = G.valid ? std::sqrt(l2) : 0.0;
     for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) {
             G.M[k][c] = G.R[c][k] / G.s[k];
@@ -268,43 +285,43 @@ Geo make_geo(const orc_scene& S, int64_t g, double thr) {
 }
 
 struct Interval {
-    double r1, r2;
-    double disc, b, c;
+    real r1, r2;
+    real disc, b, c;
     bool valid;
 };
 
 // __intersection__gaussian (forward/gaussians_aabb.cu:324-402) in FP64; the slab-independent
 // part.  valid <=> disc > 1e-7 (the reference's only pre-test); hit on [tmin,tmax] <=>
 // valid && max(tmin, r1) <= min(tmax, r2).
-Interval intersect64(const Geo& G, const double o[3], const double d[3]) {
+Interval intersect64(const Geo& G, const real o[3], const real d[3]) {
     Interval I;
     I.valid = false;
     I.r1 = I.r2 = 0.0;
     I.disc = I.b = I.c = 0.0;
     if (!G.valid) return I;
-    double O[3], Op[3], Dp[3];
+    real O[3], Op[3], Dp[3];
     for (int k = 0; k < 3; ++k) O[k] = o[k] - G.mu[k];
     for (int k = 0; k < 3; ++k) {
         Op[k] = G.Mp[k][0] * O[0] + G.Mp[k][1] * O[1] + G.Mp[k][2] * O[2];
         Dp[k] = G.Mp[k][0] * d[0] + G.Mp[k][1] * d[1] + G.Mp[k][2] * d[2];
     }
-    const double l = 1.0 / std::sqrt(Dp[0] * Dp[0] + Dp[1] * Dp[1] + Dp[2] * Dp[2]);
-    double Dh[3];
+    const real l = 1.0 / std::sqrt(Dp[0] * Dp[0] + Dp[1] * Dp[1] + Dp[2] * Dp[2]);
+    real Dh[3];
     for (int k = 0; k < 3; ++k) Dh[k] = Dp[k] * l;
-    const double b = -(Op[0] * Dh[0] + Op[1] * Dh[1] + Op[2] * Dh[2]);
-    const double c = Op[0] * Op[0] + Op[1] * Op[1] + Op[2] * Op[2] - 1.0;
-    double pp = 0.0;
+    const real b = -(Op[0] * Dh[0] + Op[1] * Dh[1] + Op[2] * Dh[2]);
+    const real c = Op[0] * Op[0] + Op[1] * Op[1] + Op[2] * Op[2] - 1.0;
+    real pp = 0.0;
     for (int k = 0; k < 3; ++k) {
-        const double p = Op[k] + b * Dh[k];
+        const real p = Op[k] + b * Dh[k];
         pp += p * p;
     }
-    const double disc = 1.0 - pp;
+    const real disc = 1.0 - pp;
     I.disc = disc;
     I.b = b;
     I.c = c;
     if (disc > 1e-7) {
-        const double sd = std::sqrt(disc);
-        const double q = (b > 0.0) ? b + sd : b - sd;
+        const real sd = std::sqrt(disc);
+        const real q = (b > 0.0) ? b + sd : b - sd;
         I.r1 = (c / q) * l;
         I.r2 = q * l;
         I.valid = true;
@@ -312,14 +329,14 @@ Interval intersect64(const Geo& G, const double o[3], const double d[3]) {
     return I;
 }
 
-inline bool overlap(double tmin, double tmax, double r1, double r2) {
+inline bool overlap(real tmin, real tmax, real r1, real r2) {
     return std::fmax(tmin, r1) <= std::fmin(tmax, r2);
 }
 
 // ---------------------------------------------------------------- CPU BVH (oracle only) -----
 
 struct BNode {
-    double lo[3], hi[3];
+    real lo[3], hi[3];
     int64_t left, right;   // >=0 internal; leaf when left < 0: prims [first, first+count)
     int64_t first, count;
 };
@@ -329,7 +346,7 @@ struct CpuBVH {
     std::vector<int64_t> prims;
 };
 
-void bvh_build_rec(CpuBVH& B, const std::vector<double>& box, std::vector<int64_t>& idx, int64_t lo,
+void bvh_build_rec(CpuBVH& B, const std::vector<real>& box, std::vector<int64_t>& idx, int64_t lo,
                    int64_t hi, int64_t node) {
     BNode nd;
     for (int k = 0; k < 3; ++k) {
@@ -337,7 +354,7 @@ void bvh_build_rec(CpuBVH& B, const std::vector<double>& box, std::vector<int64_
         nd.hi[k] = -INFINITY;
     }
     for (int64_t i = lo; i < hi; ++i) {
-        const double* b = &box[6 * idx[i]];
+        const real* b = &box[6 * idx[i]];
         for (int k = 0; k < 3; ++k) {
             nd.lo[k] = std::min(nd.lo[k], b[k]);
             nd.hi[k] = std::max(nd.hi[k], b[3 + k]);
@@ -350,11 +367,11 @@ void bvh_build_rec(CpuBVH& B, const std::vector<double>& box, std::vector<int64_
         B.nodes[node] = nd;
         return;
     }
-    double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    real clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = lo; i < hi; ++i) {
-        const double* b = &box[6 * idx[i]];
+        const real* b = &box[6 * idx[i]];
         for (int k = 0; k < 3; ++k) {
-            const double cc = 0.5 * (b[k] + b[3 + k]);
+            const real cc = 0.5 * (b[k] + b[3 + k]);
             clo[k] = std::min(clo[k], cc);
             chi[k] = std::max(chi[k], cc);
         }
@@ -382,7 +399,7 @@ void bvh_build_rec(CpuBVH& B, const std::vector<double>& box, std::vector<int64_
 // true outside it.  Invalid Gaussians get no box (they can never be hit).
 CpuBVH bvh_build(const std::vector<Geo>& geo) {
     const int64_t n = (int64_t)geo.size();
-    std::vector<double> box(6 * n);
+    std::vector<real> box(6 * n);
     std::vector<int64_t> idx;
     idx.reserve(n);
     for (int64_t g = 0; g < n; ++g) {
@@ -391,15 +408,16 @@ CpuBVH bvh_build(const std::vector<Geo>& geo) {
         for (int k = 0; k < 3; ++k) {
             // half extent along world axis k = || row k of (R * S') ||, S' = S*delta.
             // Mp rows are col_j(R)/s'_j, so s'_j = 1/||Mp row j||.
-            double h2 = 0.0;
+            real h2 = 0.0;
             for (int j = 0; j < 3; ++j) {
-                const double nrm = std::sqrt(G.Mp[j][0] * G.Mp[j][0] + G.Mp[j][1] * G.Mp[j][1] +
+                const real nrm = std::sqrt(G.Mp[j][0] * G.Mp[j][0] + G.Mp[j][1] * G.Mp[j][1] +
                                              G.Mp[j][2] * G.Mp[j][2]);
-                const double sp = 1.0 / nrm;
-                const double e = G.R[k][j] * sp;
+                const real sp = 1.0 / nrm;
+                const real e = G.R[k][j] * sp;
                 h2 += e * e;
             }
-            const double h = std::sqrt(h2) * (1.0 + 1e-9) + 1e-9 * (1.0 + std::fabs(G.mu[k]));
+            const real pad = kOracle ? 1e-9 : 1e-5;  // relative to the type's rounding
+            const real h = std::sqrt(h2) * (1.0 + pad) + pad * (1.0 + std::fabs(G.mu[k]));
             box[6 * g + k] = G.mu[k] - h;
             box[6 * g + 3 + k] = G.mu[k] + h;
         }
@@ -423,11 +441,11 @@ CpuBVH bvh_build(const std::vector<Geo>& geo) {
     return B;
 }
 
-bool box_seg(const BNode& nd, const double o[3], const double invd[3], double t0, double t1) {
-    double tn = t0, tf = t1;
+bool box_seg(const BNode& nd, const real o[3], const real invd[3], real t0, real t1) {
+    real tn = t0, tf = t1;
     for (int k = 0; k < 3; ++k) {
-        double a = (nd.lo[k] - o[k]) * invd[k];
-        double b = (nd.hi[k] - o[k]) * invd[k];
+        real a = (nd.lo[k] - o[k]) * invd[k];
+        real b = (nd.hi[k] - o[k]) * invd[k];
         if (std::isnan(a) || std::isnan(b)) {
             // ray parallel to the slab with origin on its boundary: inside iff lo<=o<=hi
             if (o[k] < nd.lo[k] || o[k] > nd.hi[k]) return false;
@@ -437,14 +455,15 @@ bool box_seg(const BNode& nd, const double o[3], const double invd[3], double t0
         tn = std::max(tn, a);
         tf = std::min(tf, b);
     }
-    return tn <= tf * (1.0 + 1e-12) + 1e-12;
+    const real tol = kOracle ? 1e-12 : 1e-6;
+    return tn <= tf * (1.0 + tol) + tol;
 }
 
-void bvh_query(const CpuBVH& B, const double o[3], const double d[3], double t0, double t1,
+void bvh_query(const CpuBVH& B, const real o[3], const real d[3], real t0, real t1,
                std::vector<int64_t>& out) {
     out.clear();
     if (B.prims.empty()) return;
-    double invd[3];
+    real invd[3];
     for (int k = 0; k < 3; ++k) invd[k] = 1.0 / d[k];
     int64_t stack[128];
     int sp = 0;
@@ -463,22 +482,22 @@ void bvh_query(const CpuBVH& B, const double o[3], const double d[3], double t0,
 
 // ---------------------------------------------------------------- SH / SG ------------------
 
-const double kC0 = 0.28209479177387814;
-const double kC1 = 0.4886025119029199;
-const double kC2[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792,
+const real kC0 = 0.28209479177387814;
+const real kC1 = 0.4886025119029199;
+const real kC2[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792,
                        0.5462742152960396};
-const double kC3[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154,
+const real kC3[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154,
                        -0.4570457994644658, 1.445305721320277, -0.5900435899266435};
 
 // Real SH basis in the reference's order and signs (utils/sh_utils.cu:60-117).
-void sh_basis(double x, double y, double z, int deg, double* Y) {
+void sh_basis(real x, real y, real z, int deg, real* Y) {
     Y[0] = kC0;
     if (deg < 1) return;
     Y[1] = -kC1 * y;
     Y[2] = kC1 * z;
     Y[3] = -kC1 * x;
     if (deg < 2) return;
-    const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    const real xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
     Y[4] = kC2[0] * xy;
     Y[5] = kC2[1] * yz;
     Y[6] = kC2[2] * (2.0 * zz - xx - yy);
@@ -497,7 +516,7 @@ void sh_basis(double x, double y, double z, int deg, double* Y) {
 // dY/dx, dY/dy, dY/dz as used by backward_sh (utils/sh_utils.cu:176-303).  With exact=0 the
 // d/dy of Y9 is C3[0]*(3xx - yy), as the reference writes it (sh_utils.cu:254); the exact
 // derivative is C3[0]*3*(xx - yy).
-void sh_basis_grad(double x, double y, double z, int deg, int exact, double* dX, double* dY, double* dZ) {
+void sh_basis_grad(real x, real y, real z, int deg, int exact, real* dX, real* dY, real* dZ) {
     const int K = (deg + 1) * (deg + 1);
     for (int k = 0; k < K; ++k) dX[k] = dY[k] = dZ[k] = 0.0;
     if (deg < 1) return;
@@ -505,7 +524,7 @@ void sh_basis_grad(double x, double y, double z, int deg, int exact, double* dX,
     dZ[2] = kC1;
     dX[3] = -kC1;
     if (deg < 2) return;
-    const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    const real xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
     dX[4] = kC2[0] * y;
     dY[4] = kC2[0] * x;
     dY[5] = kC2[1] * z;
@@ -701,19 +720,19 @@ float sigw32(const Dec32& D, int64_t g, const Ray32& r, float t) {
 
 // Decision-matched mode: a sample decision within this relative distance of the threshold may be
 // taken differently by the GPU's native exp2 (about one ulp from the host's).
-const double kEpsDec = 1e-6;
+const real kEpsDec = 1e-6;
 
 // Fragility kinds (RayOut.fragile bits): near-threshold sample decisions, a transmittance check
 // within rounding of T_eps, the origin-inside branch decided by a b within rounding of zero.
 enum { kFragSample = 1, kFragExit = 2, kFragQuirk = 4 };
 
 struct RayOut {
-    double C[3];
-    double depth;
-    double T;
+    real C[3];
+    real depth;
+    real T;
     int fragile;      // kFrag* bits
     int64_t nfrag;    // sample decisions (hit or near Gaussians) with |sigma*w/thr - 1| <= eps_w
-    double cmax;      // largest |colour component| among the ray's hit Gaussians and the background
+    real cmax;      // largest |colour component| among the ray's hit Gaussians and the background
     int64_t K;        // slab interactions (sum over slabs of |hit set|)
     int64_t P;        // distinct (ray, Gaussian) pairs among hit sets
     int64_t maxhits;  // largest per-slab hit set
@@ -721,7 +740,7 @@ struct RayOut {
     int64_t nc;       // contributing (sample, hit) events: sigma*w > thr
 };
 
-inline bool near_rel(double a, double b, double rel) {
+inline bool near_rel(real a, real b, real rel) {
     return std::fabs(a - b) <= rel * std::max(std::fabs(a), std::fabs(b));
 }
 
@@ -742,36 +761,40 @@ inline bool near_rel(double a, double b, double rel) {
 // d <= thr (1 + eps_w), which moves C by at most d dt (3 cmax + max(cmax, bg)) (the sample's
 // opacity, its mixed colour and the transmittance of everything behind it), i.e. at most
 // 4 thr (1 + eps_w) dt max(cmax, 1) per decision.
-const double kEpsW = 1e-4;  // default relative margin on sigma*w
-const double kEpsD = 5e-5;  // absolute, on disc
-inline double eps_t(double t) { return 1e-6 * std::max(1.0, std::fabs(t)); }
+const real kEpsW = 1e-4;  // default relative margin on sigma*w
+const real kEpsD = 5e-5;  // absolute, on disc
+inline real eps_t(real t) { return 1e-6 * std::max(1.0, std::fabs(t)); }
 
 struct Hit {
     int64_t g;
-    double r1, r2;
+    real r1, r2;
     bool near;  // predicate fails by less than kEpsD: evaluated for fragility only
 };
 
 // Collects every Gaussian whose predicate interval overlaps [tenter, texit] (and the near
 // misses), sorted by (r1, g).
-void gather(const CpuBVH& B, const std::vector<Geo>& geo, const double o[3], const double d[3], double t0,
-            double t1, std::vector<Hit>& hits, std::vector<int64_t>& scratch, int* fragile) {
+void gather(const CpuBVH& B, const std::vector<Geo>& geo, const real o[3], const real d[3], real t0,
+            real t1, std::vector<Hit>& hits, std::vector<int64_t>& scratch, int* fragile) {
     // fragile: kFragQuirk is or-ed in
     hits.clear();
     bvh_query(B, o, d, t0 - eps_t(t0), t1 + eps_t(t1), scratch);
     for (int64_t g : scratch) {
         const Interval I = intersect64(geo[g], o, d);
+        if (!kOracle) {  // baseline: hits only, no margins
+            if (I.valid && overlap(t0, t1, I.r1, I.r2)) hits.push_back(Hit{g, I.r1, I.r2, false});
+            continue;
+        }
         // the b > 0 branch decides hit / no hit when the origin is inside the ellipsoid
         if (I.c < 0.0 && std::fabs(I.b) < 1e-5) *fragile |= kFragQuirk;
         Hit h{g, I.r1, I.r2, false};
         if (!I.valid) {
             if (!(geo[g].valid && I.disc > 1e-7 - kEpsD && I.b > 0.0)) continue;
             // near miss: interval of the ellipsoid it grazes
-            const double sd = std::sqrt(std::max(I.disc, 0.0));
-            const double q = I.b + sd;
-            double l = 0.0;
+            const real sd = std::sqrt(std::max(I.disc, 0.0));
+            const real q = I.b + sd;
+            real l = 0.0;
             {
-                double Dp[3];
+                real Dp[3];
                 for (int k = 0; k < 3; ++k)
                     Dp[k] = geo[g].Mp[k][0] * d[0] + geo[g].Mp[k][1] * d[1] + geo[g].Mp[k][2] * d[2];
                 l = 1.0 / std::sqrt(Dp[0] * Dp[0] + Dp[1] * Dp[1] + Dp[2] * Dp[2]);
@@ -797,10 +820,10 @@ struct SlabWalker {
     std::vector<int> active;  // indices into hits
     std::vector<int> cur;
     std::vector<int> near;
-    void step(double tmin, double tmax, int* fragile) {
+    void step(real tmin, real tmax, int* fragile) {
         (void)fragile;
         const std::vector<Hit>& H = *hits;
-        const double emin = eps_t(tmin), emax = eps_t(tmax);
+        const real emin = eps_t(tmin), emax = eps_t(tmax);
         while (next < H.size() && H[next].r1 <= tmax + emax) active.push_back((int)next++);
         size_t w = 0;
         for (size_t a = 0; a < active.size(); ++a)
@@ -818,7 +841,7 @@ struct SlabWalker {
 };
 
 struct SampleBuf {
-    std::vector<double> sig, c0, c1, c2;
+    std::vector<real> sig, c0, c1, c2;
     void reset(int spb) {
         sig.assign(spb, 0.0);
         c0.assign(spb, 0.0);
@@ -827,45 +850,45 @@ struct SampleBuf {
     }
 };
 
-inline double gauss_w(const Geo& G, const double x[3], double y[3]) {
-    double X[3];
+inline real gauss_w(const Geo& G, const real x[3], real y[3]) {
+    real X[3];
     for (int k = 0; k < 3; ++k) X[k] = x[k] - G.mu[k];
     for (int k = 0; k < 3; ++k) y[k] = G.M[k][0] * X[0] + G.M[k][1] * X[1] + G.M[k][2] * X[2];
     return std::exp(-0.5 * (y[0] * y[0] + y[1] * y[1] + y[2] * y[2]));
 }
 
 // computeBufferForward (forward/gaussians_aabb.cu:437-498) for one slab, FP64.
-void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const double* rgb,
+void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const real* rgb,
                      const std::vector<Hit>& H, const std::vector<int>& cur, const Ray32& r, const SlabSamples& ts,
                      SampleBuf& buf, int64_t* nfrag, int64_t* nc = nullptr, const std::vector<int>* near = nullptr,
                      const Dec32* dec = nullptr) {
-    const double thr = p.sigma_thr;
-    const double epsw = p.eps_w > 0.0f ? (double)p.eps_w : kEpsW;
+    const real thr = p.sigma_thr;
+    const real epsw = p.eps_w > 0.0f ? (real)p.eps_w : kEpsW;
     if (near)
         for (int a : *near) {
             const Geo& G = geo[H[a].g];
             for (int i = 0; i < p.spb; ++i) {
-                const double t = ts.t[i];
-                double x[3], y[3];
-                for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
+                const real t = ts.t[i];
+                real x[3], y[3];
+                for (int k = 0; k < 3; ++k) x[k] = (real)r.o[k] + t * (real)r.d[k];
                 if (G.sigma * gauss_w(G, x, y) >= thr * (1.0 - epsw)) ++*nfrag;
             }
         }
     for (int a : cur) {
         const Geo& G = geo[H[a].g];
         for (int i = 0; i < p.spb; ++i) {
-            const double t = ts.t[i];
-            double x[3], y[3];
-            for (int k = 0; k < 3; ++k) x[k] = (double)r.o[k] + t * (double)r.d[k];
-            const double w = gauss_w(G, x, y);
-            const double sw = G.sigma * w;
+            const real t = ts.t[i];
+            real x[3], y[3];
+            for (int k = 0; k < 3; ++k) x[k] = (real)r.o[k] + t * (real)r.d[k];
+            const real w = gauss_w(G, x, y);
+            const real sw = G.sigma * w;
             bool take;
             if (dec) {  // the HIP path's FP32 decision, the FP64 value
                 const float v = sigw32(*dec, H[a].g, r, ts.t[i]);
-                if (std::fabs((double)v / (double)dec->thr - 1.0) <= kEpsDec) ++*nfrag;
+                if (std::fabs((real)v / (real)dec->thr - 1.0) <= kEpsDec) ++*nfrag;
                 take = v > dec->thr;
             } else {
-                if (std::fabs(sw / thr - 1.0) <= epsw) ++*nfrag;
+                if (kOracle && std::fabs(sw / thr - 1.0) <= epsw) ++*nfrag;
                 take = sw > thr;
             }
             if (take) {
@@ -881,7 +904,7 @@ void accumulate_slab(const orc_params& p, const std::vector<Geo>& geo, const dou
 
 struct RayCtx {
     Ray32 r;
-    double o[3], d[3];
+    real o[3], d[3];
     float tenter, texit;
     uint32_t seed;  // generator state after ray generation
 };
@@ -917,13 +940,13 @@ const std::vector<int>& dec_members(const Dec32& D, const std::vector<Hit>& H, c
 }
 
 // __raygen__rg forward (forward/gaussians_aabb.cu:521-612) + colorBlending (:500-519), FP64.
-RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const CpuBVH& B, const double* rgb,
+RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const CpuBVH& B, const real* rgb,
                      const RayCtx& c, std::vector<Hit>& hits, std::vector<int64_t>& scratch,
                      const Dec32* dec = nullptr) {
     RayOut out;
     std::memset(&out, 0, sizeof(out));
-    const double bg = p.white_bg ? 1.0 : 0.0;
-    double T = 1.0;
+    const real bg = p.white_bg ? 1.0 : 0.0;
+    real T = 1.0;
     if (!(c.tenter < c.texit)) {
         for (int k = 0; k < 3; ++k) out.C[k] = bg;
         out.T = 1.0;
@@ -938,8 +961,8 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
     W.hits = &hits;
     SampleBuf buf;
     const float slab = p.dt * (float)p.spb;
-    const double dt = p.dt;
-    const double eps = p.t_eps;
+    const real dt = p.dt;
+    const real eps = p.t_eps;
     float tb = c.tenter;
     uint32_t seed = c.seed;
     std::vector<char> seen(hits.size(), 0);
@@ -970,13 +993,13 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
                 accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &out.nfrag, &out.nc, dec ? nullptr : &W.near,
                                 dec);
                 for (int i = 0; i < p.spb; ++i) {
-                    const double s = buf.sig[i];
-                    const double alpha = 1.0 - std::exp(-s * dt);
-                    double ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
+                    const real s = buf.sig[i];
+                    const real alpha = 1.0 - std::exp(-s * dt);
+                    real ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
                     if (s > 0.0)
                         for (int k = 0; k < 3; ++k) ch[k] /= s;
                     for (int k = 0; k < 3; ++k) out.C[k] += T * alpha * ch[k];
-                    out.depth += T * alpha * (double)ts.t[i];
+                    out.depth += T * alpha * (real)ts.t[i];
                     T *= 1.0 - alpha;
                 }
                 seed = after_slab(p, seed);
@@ -998,11 +1021,11 @@ RayOut march_forward(const orc_params& p, const std::vector<Geo>& geo, const Cpu
 
 // Literal restatement: every Gaussian's predicate is evaluated in every slab (no interval
 // reuse, no BVH).  Quadratic cost; for pinning march_forward on small scenes only.
-RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, const double* rgb, const RayCtx& c) {
+RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, const real* rgb, const RayCtx& c) {
     RayOut out;
     std::memset(&out, 0, sizeof(out));
-    const double bg = p.white_bg ? 1.0 : 0.0;
-    double T = 1.0;
+    const real bg = p.white_bg ? 1.0 : 0.0;
+    real T = 1.0;
     if (!(c.tenter < c.texit)) {
         for (int k = 0; k < 3; ++k) out.C[k] = bg;
         out.T = 1.0;
@@ -1014,7 +1037,7 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
     std::vector<Hit> hits;
     std::vector<int> cur;
     SampleBuf buf;
-    while (tb < c.texit && T > (double)p.t_eps) {
+    while (tb < c.texit && T > (real)p.t_eps) {
         const float tmin = std::fmax(c.tenter, tb);
         const float tmax = std::fmin(c.texit, tb + slab);
         out.slabs++;
@@ -1033,13 +1056,13 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
                 const SlabSamples ts(p, tb, seed);
                 accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &dummy);
                 for (int i = 0; i < p.spb; ++i) {
-                    const double s = buf.sig[i];
-                    const double alpha = 1.0 - std::exp(-s * (double)p.dt);
-                    double ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
+                    const real s = buf.sig[i];
+                    const real alpha = 1.0 - std::exp(-s * (real)p.dt);
+                    real ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
                     if (s > 0.0)
                         for (int k = 0; k < 3; ++k) ch[k] /= s;
                     for (int k = 0; k < 3; ++k) out.C[k] += T * alpha * ch[k];
-                    out.depth += T * alpha * (double)ts.t[i];
+                    out.depth += T * alpha * (real)ts.t[i];
                     T *= 1.0 - alpha;
                 }
                 seed = after_slab(p, seed);
@@ -1052,15 +1075,15 @@ RayOut march_forward_literal(const orc_params& p, const std::vector<Geo>& geo, c
     return out;
 }
 
-inline void atomic_add(double* p, double v) {
+inline void atomic_add(real* p, real v) {
 #pragma omp atomic
     *p += v;
 }
 
 // __raygen__rg backward (backward/gaussians_aabb.cu:505-775), FP64, literal per-hit formulas.
-void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuBVH& B, const double* rgb,
-                    const RayCtx& c, const double* Cfwd, const double* dLdC, double* dpos, double* dscale,
-                    double* dquat, double* ddens, double* drgb, const orc_scene& S, std::vector<Hit>& hits,
+void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuBVH& B, const real* rgb,
+                    const RayCtx& c, const real* Cfwd, const real* dLdC, real* dpos, real* dscale,
+                    real* dquat, real* ddens, real* drgb, const orc_scene& S, std::vector<Hit>& hits,
                     std::vector<int64_t>& scratch, const Dec32* dec = nullptr) {
     if (!(c.tenter < c.texit)) return;
     int fragile = 0;
@@ -1069,16 +1092,16 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
     W.hits = &hits;
     SampleBuf buf;
     const float slab = p.dt * (float)p.spb;
-    const double dt = p.dt;
-    const double thr = p.sigma_thr;
-    double T = 1.0;
-    double Crem[3] = {Cfwd[0], Cfwd[1], Cfwd[2]};
+    const real dt = p.dt;
+    const real thr = p.sigma_thr;
+    real T = 1.0;
+    real Crem[3] = {Cfwd[0], Cfwd[1], Cfwd[2]};
     float tb = c.tenter;
     uint32_t seed = c.seed;
     const int spb = p.spb;
-    std::vector<double> alpha(spb), chat(3 * spb);
+    std::vector<real> alpha(spb), chat(3 * spb);
     std::vector<int> memv;
-    while (tb < c.texit && T > (double)p.t_eps) {
+    while (tb < c.texit && T > (real)p.t_eps) {
         const float tmin = std::fmax(c.tenter, tb);
         const float tmax = std::fmin(c.texit, tb + slab);
         if (tmax > c.tenter) {
@@ -1091,37 +1114,37 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
                 int64_t nfrag = 0;
                 accumulate_slab(p, geo, rgb, hits, cur, c.r, ts, buf, &nfrag, nullptr, nullptr, dec);
                 for (int i = 0; i < spb; ++i) {
-                    const double s = buf.sig[i];
-                    double ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
+                    const real s = buf.sig[i];
+                    real ch[3] = {buf.c0[i], buf.c1[i], buf.c2[i]};
                     if (s > 0.0)
                         for (int k = 0; k < 3; ++k) ch[k] /= s;
                     for (int k = 0; k < 3; ++k) chat[3 * i + k] = ch[k];
                     alpha[i] = 1.0 - std::exp(-s * dt);
                 }
-                double Tn = T, Cn[3] = {Crem[0], Crem[1], Crem[2]};
+                real Tn = T, Cn[3] = {Crem[0], Crem[1], Crem[2]};
                 for (int a : cur) {
                     const int64_t g = hits[a].g;
                     const Geo& G = geo[g];
-                    double Ta = T;
-                    double Ca[3] = {Crem[0], Crem[1], Crem[2]};
-                    double grad_color = 0.0;
-                    double dcds[3] = {0, 0, 0};
-                    double dsc[3] = {0, 0, 0}, dps[3] = {0, 0, 0};
-                    double dr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dr[k] = dL/dcol_k
-                    const double* col = rgb + 3 * g;
+                    real Ta = T;
+                    real Ca[3] = {Crem[0], Crem[1], Crem[2]};
+                    real grad_color = 0.0;
+                    real dcds[3] = {0, 0, 0};
+                    real dsc[3] = {0, 0, 0}, dps[3] = {0, 0, 0};
+                    real dr[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dr[k] = dL/dcol_k
+                    const real* col = rgb + 3 * g;
                     for (int i = 0; i < spb; ++i) {
-                        const double cd = buf.sig[i];
-                        const double* cc = &chat[3 * i];
-                        const double al = alpha[i];
-                        const double t = ts.t[i];
-                        double x[3], y[3];
-                        for (int k = 0; k < 3; ++k) x[k] = (double)c.r.o[k] + t * (double)c.r.d[k];
-                        const double w = gauss_w(G, x, y);
+                        const real cd = buf.sig[i];
+                        const real* cc = &chat[3 * i];
+                        const real al = alpha[i];
+                        const real t = ts.t[i];
+                        real x[3], y[3];
+                        for (int k = 0; k < 3; ++k) x[k] = (real)c.r.o[k] + t * (real)c.r.d[k];
+                        const real w = gauss_w(G, x, y);
                         if (dec ? sigw32(*dec, g, c.r, ts.t[i]) > dec->thr : G.sigma * w > thr) {
-                            double wn = w * G.sigma;
+                            real wn = w * G.sigma;
                             if (cd > 0.0) wn /= cd;
                             grad_color += wn * al * Ta;
-                            double aux[3], dcdw[3];
+                            real aux[3], dcdw[3];
                             for (int k = 0; k < 3; ++k) aux[k] = (Ta * cc[k] - Ca[k]) * dt;
                             for (int k = 0; k < 3; ++k) {
                                 dcds[k] += aux[k] * w;
@@ -1132,13 +1155,13 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
                                     dcdw[k] += Ta * al * ((col[k] - cc[k]) / cd) * G.sigma;
                                     dcds[k] += Ta * al * ((col[k] - cc[k]) / cd) * w;
                                 }
-                            const double dLdw = dLdC[0] * dcdw[0] + dLdC[1] * dcdw[1] + dLdC[2] * dcdw[2];
-                            double X[3];
+                            const real dLdw = dLdC[0] * dcdw[0] + dLdC[1] * dcdw[1] + dLdC[2] * dcdw[2];
+                            real X[3];
                             for (int k = 0; k < 3; ++k) X[k] = x[k] - G.mu[k];
                             for (int k = 0; k < 3; ++k) dsc[k] += dLdw * w * (y[k] * y[k]) / G.s[k];
                             for (int k = 0; k < 3; ++k) {
                                 // dot(M_k, y) with M_k = column k of M (coefficients of X[k])
-                                const double m = G.M[0][k] * y[0] + G.M[1][k] * y[1] + G.M[2][k] * y[2];
+                                const real m = G.M[0][k] * y[0] + G.M[1][k] * y[1] + G.M[2][k] * y[2];
                                 dps[k] += dLdw * w * m;
                             }
                             for (int kk = 0; kk < 3; ++kk)
@@ -1153,17 +1176,17 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
                     for (int k = 0; k < 3; ++k) atomic_add(&dpos[3 * g + k], dps[k]);
                     // quaternion chain (backward/gaussians_aabb.cu:747-763), q = (r,i,j,k)
                     const float* q = S.quat + 4 * g;
-                    const double r = q[0], qi = q[1], qj = q[2], qk = q[3];
-                    const double* g0 = dr[0];
-                    const double* g1 = dr[1];
-                    const double* g2 = dr[2];
-                    const double dq_r = 2 * qk * g0[1] - 2 * qj * g0[2] - 2 * qk * g1[0] + 2 * qi * g1[2] +
+                    const real r = q[0], qi = q[1], qj = q[2], qk = q[3];
+                    const real* g0 = dr[0];
+                    const real* g1 = dr[1];
+                    const real* g2 = dr[2];
+                    const real dq_r = 2 * qk * g0[1] - 2 * qj * g0[2] - 2 * qk * g1[0] + 2 * qi * g1[2] +
                                         2 * qj * g2[0] - 2 * qi * g2[1];
-                    const double dq_i = 2 * qj * g0[1] + 2 * qk * g0[2] + 2 * qj * g1[0] - 4 * qi * g1[1] +
+                    const real dq_i = 2 * qj * g0[1] + 2 * qk * g0[2] + 2 * qj * g1[0] - 4 * qi * g1[1] +
                                         2 * r * g1[2] + 2 * qk * g2[0] - 2 * r * g2[1] - 4 * qi * g2[2];
-                    const double dq_j = -4 * qj * g0[0] + 2 * qi * g0[1] - 2 * r * g0[2] + 2 * qi * g1[0] +
+                    const real dq_j = -4 * qj * g0[0] + 2 * qi * g0[1] - 2 * r * g0[2] + 2 * qi * g1[0] +
                                         2 * qk * g1[2] + 2 * r * g2[0] + 2 * qk * g2[1] - 4 * qj * g2[2];
-                    const double dq_k = -4 * qk * g0[0] + 2 * r * g0[1] + 2 * qi * g0[2] - 2 * r * g1[0] -
+                    const real dq_k = -4 * qk * g0[0] + 2 * r * g0[1] + 2 * qi * g0[2] - 2 * r * g1[0] -
                                         4 * qk * g1[1] + 2 * qj * g1[2] + 2 * qi * g2[0] + 2 * qj * g2[1];
                     atomic_add(&dquat[4 * g + 0], dq_r);
                     atomic_add(&dquat[4 * g + 1], dq_i);
@@ -1182,7 +1205,7 @@ void march_backward(const orc_params& p, const std::vector<Geo>& geo, const CpuB
     }
 }
 
-std::vector<Geo> make_geos(const orc_scene& S, double thr, int nthreads) {
+std::vector<Geo> make_geos(const orc_scene& S, real thr, int nthreads) {
     std::vector<Geo> geo(S.n);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t g = 0; g < S.n; ++g) geo[g] = make_geo(S, g, thr);
@@ -1192,6 +1215,8 @@ std::vector<Geo> make_geos(const orc_scene& S, double thr, int nthreads) {
 }  // namespace
 
 extern "C" {
+
+#ifndef ORC_BASELINE
 
 int orc_version() { return 1; }
 
@@ -1244,24 +1269,24 @@ void orc_ray_interval(const orc_camera* cam, int64_t nrays, const int32_t* pixel
 }
 
 // forward_sh (utils/sh_utils.cu:25-119), FP64.  rgb: (n,3).
-void orc_sh_forward(const orc_scene* S, const float* eye, double* rgb, int nthreads) {
+void orc_sh_forward(const orc_scene* S, const float* eye, real* rgb, int nthreads) {
     const int K = (S->deg + 1) * (S->deg + 1);
     const int L = S->L;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t g = 0; g < S->n; ++g) {
-        double dir[3];
-        for (int k = 0; k < 3; ++k) dir[k] = (double)S->pos[3 * g + k] - (double)eye[k];
-        const double len = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        real dir[3];
+        for (int k = 0; k < 3; ++k) dir[k] = (real)S->pos[3 * g + k] - (real)eye[k];
+        const real len = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
         for (int k = 0; k < 3; ++k) dir[k] /= len;
-        double Y[16];
+        real Y[16];
         sh_basis(dir[0], dir[1], dir[2], S->deg, Y);
         for (int ch = 0; ch < 3; ++ch) {
-            double acc = 0.0;
+            real acc = 0.0;
             for (int l = 0; l < L; ++l) {
-                const double a = S->sg[g * 3 * L + ch * L + l];
-                const double lam = S->lam[g * L + l];
+                const real a = S->sg[g * 3 * L + ch * L + l];
+                const real lam = S->lam[g * L + l];
                 const float* ax = S->axis + g * 3 * L + 3 * l;
-                const double dp = dir[0] * ax[0] + dir[1] * ax[1] + dir[2] * ax[2];
+                const real dp = dir[0] * ax[0] + dir[1] * ax[1] + dir[2] * ax[2];
                 acc += a * std::exp(lam * (dp - 1.0));
             }
             for (int k = 0; k < K; ++k) acc += S->sh[g * 3 * K + ch * K + k] * Y[k];
@@ -1271,35 +1296,35 @@ void orc_sh_forward(const orc_scene* S, const float* eye, double* rgb, int nthre
 }
 
 // backward_sh (utils/sh_utils.cu:121-315), FP64.  Outputs are overwritten.
-void orc_sh_backward(const orc_scene* S, const float* eye, const double* dLdrgb, int exact, double* dsh,
-                     double* dsg, double* dlam, double* daxis, double* dpos_dir, int nthreads) {
+void orc_sh_backward(const orc_scene* S, const float* eye, const real* dLdrgb, int exact, real* dsh,
+                     real* dsg, real* dlam, real* daxis, real* dpos_dir, int nthreads) {
     const int K = (S->deg + 1) * (S->deg + 1);
     const int L = S->L;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t g = 0; g < S->n; ++g) {
-        const double* dL = dLdrgb + 3 * g;
-        double d0[3];
-        for (int k = 0; k < 3; ++k) d0[k] = (double)S->pos[3 * g + k] - (double)eye[k];
-        const double len = std::sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
-        double dir[3] = {d0[0] / len, d0[1] / len, d0[2] / len};
-        double Y[16], dX[16], dY[16], dZ[16];
+        const real* dL = dLdrgb + 3 * g;
+        real d0[3];
+        for (int k = 0; k < 3; ++k) d0[k] = (real)S->pos[3 * g + k] - (real)eye[k];
+        const real len = std::sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+        real dir[3] = {d0[0] / len, d0[1] / len, d0[2] / len};
+        real Y[16], dX[16], dY[16], dZ[16];
         sh_basis(dir[0], dir[1], dir[2], S->deg, Y);
         sh_basis_grad(dir[0], dir[1], dir[2], S->deg, exact, dX, dY, dZ);
-        double ddir[3] = {0, 0, 0};
+        real ddir[3] = {0, 0, 0};
         for (int ch = 0; ch < 3; ++ch)
             for (int k = 0; k < K; ++k) {
                 dsh[g * 3 * K + ch * K + k] = Y[k] * dL[ch];
-                const double c = S->sh[g * 3 * K + ch * K + k];
+                const real c = S->sh[g * 3 * K + ch * K + k];
                 ddir[0] += dL[ch] * c * dX[k];
                 ddir[1] += dL[ch] * c * dY[k];
                 ddir[2] += dL[ch] * c * dZ[k];
             }
         for (int l = 0; l < L; ++l) {
             const float* ax = S->axis + g * 3 * L + 3 * l;
-            const double lam = S->lam[g * L + l];
-            const double dm1 = dir[0] * ax[0] + dir[1] * ax[1] + dir[2] * ax[2] - 1.0;
-            const double wsg = std::exp(dm1 * lam);
-            double dLdw = 0.0;
+            const real lam = S->lam[g * L + l];
+            const real dm1 = dir[0] * ax[0] + dir[1] * ax[1] + dir[2] * ax[2] - 1.0;
+            const real wsg = std::exp(dm1 * lam);
+            real dLdw = 0.0;
             for (int ch = 0; ch < 3; ++ch) {
                 dsg[g * 3 * L + ch * L + l] = dL[ch] * wsg;
                 dLdw += dL[ch] * S->sg[g * 3 * L + ch * L + l];
@@ -1309,10 +1334,10 @@ void orc_sh_backward(const orc_scene* S, const float* eye, const double* dLdrgb,
             for (int k = 0; k < 3; ++k) ddir[k] += dLdw * lam * wsg * ax[k];
         }
         // d normalize(d0) / d d0 = (I - dir dir^T)/|d0|
-        const double s2 = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
-        const double inv = 1.0 / (std::sqrt(s2) * s2);
+        const real s2 = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
+        const real inv = 1.0 / (std::sqrt(s2) * s2);
         for (int k = 0; k < 3; ++k) {
-            double acc = s2 * ddir[k];
+            real acc = s2 * ddir[k];
             for (int j = 0; j < 3; ++j) acc -= d0[j] * d0[k] * ddir[j];
             dpos_dir[3 * g + k] = acc * inv;
         }
@@ -1320,38 +1345,38 @@ void orc_sh_backward(const orc_scene* S, const float* eye, const double* dLdrgb,
 }
 
 // compute_ellipsoids_bbox (optix_utils.py:54-66), FP64: (n,6) = min xyz, max xyz.
-void orc_aabb(const orc_scene* S, float sigma_thr, double* box, int nthreads) {
+void orc_aabb(const orc_scene* S, float sigma_thr, real* box, int nthreads) {
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t g = 0; g < S->n; ++g) {
-        double R[3][3];
+        real R[3][3];
         quat_to_R(S->quat + 4 * g, R);
-        const double ratio = (double)S->dens[g] / (double)sigma_thr;
-        double del = std::log(ratio * ratio);
+        const real ratio = (real)S->dens[g] / (real)sigma_thr;
+        real del = std::log(ratio * ratio);
         if (del < 0.0) del = 0.0;
         del = std::sqrt(del);
         for (int k = 0; k < 3; ++k) {
             // L_k row of R, scaled component-wise by delta*s
-            double h2 = 0.0;
+            real h2 = 0.0;
             for (int j = 0; j < 3; ++j) {
-                const double e = del * (double)S->scale[3 * g + j] * R[k][j];
+                const real e = del * (real)S->scale[3 * g + j] * R[k][j];
                 h2 += e * e;
             }
-            const double h = std::sqrt(h2);
-            box[6 * g + k] = (double)S->pos[3 * g + k] - h;
-            box[6 * g + 3 + k] = (double)S->pos[3 * g + k] + h;
+            const real h = std::sqrt(h2);
+            box[6 * g + k] = (real)S->pos[3 * g + k] - h;
+            box[6 * g + 3 + k] = (real)S->pos[3 * g + k] + h;
         }
     }
 }
 
 // The FP64 predicate of every (ray, Gaussian) pair: out (nrays, n, 5) = r1, r2, disc, b, c and
 // valid (nrays, n).  For pinning the predicate on small inputs.
-void orc_intersect(const orc_scene* S, float sigma_thr, int64_t nrays, const double* o, const double* d, double* out,
+void orc_intersect(const orc_scene* S, float sigma_thr, int64_t nrays, const real* o, const real* d, real* out,
                    int32_t* valid) {
     const std::vector<Geo> geo = make_geos(*S, sigma_thr, 1);
     for (int64_t r = 0; r < nrays; ++r)
         for (int64_t g = 0; g < S->n; ++g) {
             const Interval I = intersect64(geo[g], o + 3 * r, d + 3 * r);
-            double* q = out + 5 * (r * S->n + g);
+            real* q = out + 5 * (r * S->n + g);
             q[0] = I.r1;
             q[1] = I.r2;
             q[2] = I.disc;
@@ -1363,10 +1388,10 @@ void orc_intersect(const orc_scene* S, float sigma_thr, int64_t nrays, const dou
 
 // Forward render.  rgb: per-Gaussian colour (n,3) for this view (orc_sh_forward).
 // pixels may be null (ray r -> pixel r).  Outputs per ray; any output pointer may be null.
-int orc_render(const orc_params* p, const orc_scene* S, const double* rgb, const float* bmin, const float* bmax,
-               const orc_camera* cam, int64_t nrays, const int32_t* pixels, double* out_rgb, double* out_depth,
-               double* out_opacity, int32_t* fragile, int64_t* K, int64_t* P, int64_t* maxhits, int64_t* nc,
-               int64_t* nfrag, double* cmax, const orc_record* rec, int nthreads) {
+int orc_render(const orc_params* p, const orc_scene* S, const real* rgb, const float* bmin, const float* bmax,
+               const orc_camera* cam, int64_t nrays, const int32_t* pixels, real* out_rgb, real* out_depth,
+               real* out_opacity, int32_t* fragile, int64_t* K, int64_t* P, int64_t* maxhits, int64_t* nc,
+               int64_t* nfrag, real* cmax, const orc_record* rec, int nthreads) {
     const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
     const CpuBVH B = bvh_build(geo);
     Dec32 dec{};
@@ -1400,12 +1425,12 @@ int orc_render(const orc_params* p, const orc_scene* S, const double* rgb, const
 // from tenter to texit: out[0] for the reference's FP32 evaluation, out[1] for the HIP path's
 // closed form, out[2] the number of such samples.
 void orc_sigmaw_fp32_error(const orc_params* p, const orc_scene* S, const float* bmin, const float* bmax,
-                           const orc_camera* cam, int64_t nrays, const int32_t* pixels, double band, double* out,
+                           const orc_camera* cam, int64_t nrays, const int32_t* pixels, real band, real* out,
                            int nthreads) {
     const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
     const CpuBVH B = bvh_build(geo);
-    const double thr = p->sigma_thr;
-    double e_ref = 0.0, e_closed = 0.0, count = 0.0;
+    const real thr = p->sigma_thr;
+    real e_ref = 0.0, e_closed = 0.0, count = 0.0;
 #pragma omp parallel num_threads(nthreads) reduction(max : e_ref, e_closed) reduction(+ : count)
     {
         std::vector<Hit> hits;
@@ -1430,14 +1455,14 @@ void orc_sigmaw_fp32_error(const orc_params* p, const orc_scene* S, const float*
                 for (int a : W.cur) {
                     const int64_t g = hits[a].g;
                     for (int i = 0; i < p->spb; ++i) {
-                        double x[3], y[3];
-                        for (int k = 0; k < 3; ++k) x[k] = (double)c.r.o[k] + (double)ts.t[i] * (double)c.r.d[k];
-                        const double sw = geo[g].sigma * gauss_w(geo[g], x, y);
+                        real x[3], y[3];
+                        for (int k = 0; k < 3; ++k) x[k] = (real)c.r.o[k] + (real)ts.t[i] * (real)c.r.d[k];
+                        const real sw = geo[g].sigma * gauss_w(geo[g], x, y);
                         if (!(std::fabs(sw / thr - 1.0) < band)) continue;
                         count += 1.0;
-                        e_ref = std::max(e_ref, std::fabs((double)sigw_ref32(*S, g, c.r, ts.t[i]) / sw - 1.0));
+                        e_ref = std::max(e_ref, std::fabs((real)sigw_ref32(*S, g, c.r, ts.t[i]) / sw - 1.0));
                         const float scl = sigw_closed32(*S, g, p->sigma_thr, c.r, ts.t[i]);
-                        if (scl >= 0.0f) e_closed = std::max(e_closed, std::fabs((double)scl / sw - 1.0));
+                        if (scl >= 0.0f) e_closed = std::max(e_closed, std::fabs((real)scl / sw - 1.0));
                     }
                 }
             }
@@ -1449,9 +1474,9 @@ void orc_sigmaw_fp32_error(const orc_params* p, const orc_scene* S, const float*
 }
 
 // Literal per-slab restatement (no BVH, no interval reuse): O(slabs * n) per ray.
-int orc_render_literal(const orc_params* p, const orc_scene* S, const double* rgb, const float* bmin,
+int orc_render_literal(const orc_params* p, const orc_scene* S, const real* rgb, const float* bmin,
                        const float* bmax, const orc_camera* cam, int64_t nrays, const int32_t* pixels,
-                       double* out_rgb, double* out_depth, int64_t* K, int nthreads) {
+                       real* out_rgb, real* out_depth, int64_t* K, int nthreads) {
     const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
     for (int64_t r = 0; r < nrays; ++r) {
@@ -1468,9 +1493,9 @@ int orc_render_literal(const orc_params* p, const orc_scene* S, const double* rg
 // Backward of the ray caster for one view: accumulates (+=) into the gradient arrays
 // dpos (n,3), dscale (n,3), dquat (n,4), ddens (n), drgb (n,3).  ray_colors are the forward
 // outputs (including T*bg), dLdC the incoming gradient, both (nrays,3).
-int orc_backward(const orc_params* p, const orc_scene* S, const double* rgb, const float* bmin, const float* bmax,
-                 const orc_camera* cam, int64_t nrays, const int32_t* pixels, const double* ray_colors,
-                 const double* dLdC, double* dpos, double* dscale, double* dquat, double* ddens, double* drgb,
+int orc_backward(const orc_params* p, const orc_scene* S, const real* rgb, const float* bmin, const float* bmax,
+                 const orc_camera* cam, int64_t nrays, const int32_t* pixels, const real* ray_colors,
+                 const real* dLdC, real* dpos, real* dscale, real* dquat, real* ddens, real* drgb,
                  const orc_record* rec, int nthreads) {
     const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
     const CpuBVH B = bvh_build(geo);
@@ -1482,7 +1507,7 @@ int orc_backward(const orc_params* p, const orc_scene* S, const double* rgb, con
         std::vector<int64_t> scratch;
 #pragma omp for schedule(dynamic, 16)
         for (int64_t r = 0; r < nrays; ++r) {
-            const double* dl = dLdC + 3 * r;
+            const real* dl = dLdC + 3 * r;
             if (dl[0] == 0.0 && dl[1] == 0.0 && dl[2] == 0.0) continue;
             const RayCtx c = make_ctx(*p, *cam, pixels ? pixels[r] : r, bmin, bmax);
             march_backward(*p, geo, B, rgb, c, ray_colors + 3 * r, dl, dpos, dscale, dquat, ddens, drgb, *S, hits,
@@ -1572,5 +1597,57 @@ void orc_hits_fp32(int64_t n, const float* mu, const float* Mp, const int32_t* v
         }
     }
 }
+
+#else  // ORC_BASELINE
+
+// One training pass of the ray caster over `nrays` rays of one view, in `real` arithmetic:
+// per-Gaussian geometry and the CPU BVH, the forward, the L2 loss gradient 2 (C - target), the
+// backward.  times[0..2] = seconds (omp_get_wtime) of the three phases.  rgb (n,3) per-Gaussian
+// colour of the view; out_rgb (nrays,3); gradients accumulate (+=) as in the oracle's backward.
+int orc_baseline_step(const orc_params* p, const orc_scene* S, const real* rgb, const float* bmin, const float* bmax,
+                      const orc_camera* cam, int64_t nrays, const int32_t* pixels, const real* target,
+                      real* out_rgb, real* dpos, real* dscale, real* dquat, real* ddens, real* drgb, double* times,
+                      int nthreads) {
+    const double t0 = omp_get_wtime();
+    const std::vector<Geo> geo = make_geos(*S, p->sigma_thr, nthreads);
+    const CpuBVH B = bvh_build(geo);
+    const double t1 = omp_get_wtime();
+    std::vector<real> dLdC(3 * nrays);
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<Hit> hits;
+        std::vector<int64_t> scratch;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t r = 0; r < nrays; ++r) {
+            const RayCtx c = make_ctx(*p, *cam, pixels ? pixels[r] : r, bmin, bmax);
+            const RayOut o = march_forward(*p, geo, B, rgb, c, hits, scratch);
+            for (int k = 0; k < 3; ++k) {
+                out_rgb[3 * r + k] = o.C[k];
+                dLdC[3 * r + k] = 2.0 * (o.C[k] - target[3 * r + k]);
+            }
+        }
+    }
+    const double t2 = omp_get_wtime();
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<Hit> hits;
+        std::vector<int64_t> scratch;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t r = 0; r < nrays; ++r) {
+            const RayCtx c = make_ctx(*p, *cam, pixels ? pixels[r] : r, bmin, bmax);
+            march_backward(*p, geo, B, rgb, c, out_rgb + 3 * r, dLdC.data() + 3 * r, dpos, dscale, dquat, ddens,
+                           drgb, *S, hits, scratch);
+        }
+    }
+    const double t3 = omp_get_wtime();
+    times[0] = t1 - t0;
+    times[1] = t2 - t1;
+    times[2] = t3 - t2;
+    return 0;
+}
+
+int orc_baseline_real_bytes() { return (int)sizeof(real); }
+
+#endif  // ORC_BASELINE
 
 }  // extern "C"

@@ -331,6 +331,13 @@ class RayCaster:
         """Changes whenever the handle (re)allocates device memory (graphs over it are then stale)."""
         return int(self._L.rg_alloc_epoch(self._h))
 
+    def device_bytes(self) -> dict:
+        """Device memory the handle holds: {"total": bytes, "records": the hit-list records'
+        share (rays x replay_words x 4 B + 12 B per ray)}."""
+        rec = C.c_int64()
+        total = int(self._L.rg_device_bytes(self._h, C.byref(rec)))
+        return {"total": total, "records": int(rec.value)}
+
     def check_bvh(self) -> int:
         e = C.c_int64()
         _lib.check(self._L.rg_debug_check_bvh(self._h, C.byref(e), _stream()), self._h)

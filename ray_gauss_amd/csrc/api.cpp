@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "rg_internal.h"
@@ -58,6 +59,9 @@ struct rg_handle {
     } rec_set;
     // bumped by every device (re)allocation of the handle's workspace
     int64_t alloc_epoch = 0;
+    // bytes of every live device allocation of the handle, and their sum
+    std::unordered_map<const void*, int64_t> sizes;
+    int64_t device_bytes = 0;
 };
 
 namespace {
@@ -91,15 +95,27 @@ int fail(rg_handle* h, int code, const std::string& msg) {
 template <typename T>
 int dalloc(rg_handle* h, T** p, int64_t count) {
     if (*p) {
+        if (h) {
+            auto it = h->sizes.find(*p);
+            if (it != h->sizes.end()) {
+                h->device_bytes -= it->second;
+                h->sizes.erase(it);
+            }
+        }
         hipFree(*p);
         *p = nullptr;
     }
     if (count <= 0) count = 1;
     if (h) h->alloc_epoch++;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (size_t)count);
+    const int64_t bytes = (int64_t)sizeof(T) * count;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), (size_t)bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         return fail(h, RG_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    if (h) {
+        h->sizes[*p] = bytes;
+        h->device_bytes += bytes;
     }
     return RG_OK;
 }
@@ -612,6 +628,19 @@ int rg_get_stats(rg_handle* h, rg_stats* out, void* stream) {
 }
 
 int64_t rg_alloc_epoch(const rg_handle* h) { return h ? h->alloc_epoch : -1; }
+
+int64_t rg_device_bytes(const rg_handle* h, int64_t* record_bytes) {
+    if (!h) return -1;
+    if (record_bytes) {
+        int64_t r = 0;
+        for (const void* q : {(const void*)h->rec, (const void*)h->rec_len, (const void*)h->rec_id}) {
+            auto it = h->sizes.find(q);
+            if (q && it != h->sizes.end()) r += it->second;
+        }
+        *record_bytes = r;
+    }
+    return h->device_bytes;
+}
 
 int rg_export_predicate(rg_handle* h, float* mu, float* Mp, int32_t* valid, float* hlog, void* stream) {
     if (!h || !mu || !Mp || !valid) return fail(h, RG_ERR_ARG, "null argument");
