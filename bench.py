@@ -233,6 +233,9 @@ def main():
     ap.add_argument("--bwd-ray-chunk", type=int, default=0, help="backward launch, as --ray-chunk")
     ap.add_argument("--refill-lanes", type=int, default=0,
                     help="persistent forward: hand new rays to idle lanes once this many are idle (0: whole chunks)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch configuration by the autotuner's coordinate search before timing "
+                         "(the table is reported)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 device copies of 256 MiB (reference kernel for rocprof byte counters)")
     ap.add_argument("--full-step", action="store_true",
@@ -291,15 +294,15 @@ def main():
                       replay_words=args.replay_words, tree_builder=args.tree_builder,
                       ray_chunk=args.ray_chunk, refill_lanes=args.refill_lanes,
                       bwd_ray_chunk=args.bwd_ray_chunk)
-    rc = RayCaster(wl.params, tune)
+    rc = None  # the per-view caster, made after the batched run so that its memory is not counted there
 
     def prepare(rebuild=False):
         rc.prepare(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"], t["sg"] if L else None,
                    t["sharpness"] if L else None, t["lobe_axis"] if L else None, rebuild=rebuild)
 
-    prepare(rebuild=True)
-    rc.set_views(cams)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    free0, hbm_total = torch.cuda.mem_get_info()
 
     def step_per_view():
         prepare()
@@ -315,6 +318,11 @@ def main():
 
     rc_b = RayCaster(wl.params, tune)
     trainer = BatchedTrainer(rc_b, t, cams, ray_view, ray_pixel, target, loss_kind=0, use_graph=not args.no_graph)
+    tune_table = None
+    if args.autotune:
+        from ray_gauss_amd.batch import autotune
+
+        tune, tune_table = autotune(rc_b, ray_view, ray_pixel, target, reps=2, save=False)
     for _ in range(args.warmup):
         trainer.step()
     torch.cuda.synchronize()
@@ -342,11 +350,23 @@ def main():
         b_ms.append(ev[0].elapsed_time(ev[1]))
         rc_b.zero_grads()
     fk_ms, bk_ms = float(np.mean(f_ms)), float(np.mean(b_ms))
+    # device memory of the batched step: the caster's own buffers (tree, records, accumulators),
+    # torch's peak (parameters, targets, colours, gradients) and the device-wide use
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    hb = rc_b.device_bytes()
+    memory = {"handle_bytes": hb["total"], "record_bytes": hb["records"],
+              "torch_peak_bytes": int(torch.cuda.max_memory_allocated()),
+              "device_used_bytes": int(hbm_total - free1), "device_used_before_bytes": int(hbm_total - free0),
+              "device_total_bytes": int(hbm_total)}
     del trainer, rgb_b, dL_b
 
     # ---- per-view plain launches ----
     dt_pv = None
     if not args.no_per_view:
+        rc = RayCaster(wl.params, tune)
+        prepare(rebuild=True)
+        rc.set_views(cams)
         for _ in range(args.warmup):
             step_per_view()
         torch.cuda.synchronize()
@@ -375,7 +395,9 @@ def main():
     P, K = st["pairs"], st["slab_hits"]
     B_bwd = backward_bytes(R, P)
     achieved = B_bwd / (bk_ms * 1e-3) / 1e9
-    B_step = step_bytes(R, P, sc.n, (sc.sh_degree + 1) ** 2, L, len(cams))
+    # hit-list records: written by the forward, read back by the backward (4 B a word, each way)
+    B_rec = 8.0 * st["replay_words"]
+    B_step = step_bytes(R, P, sc.n, (sc.sh_degree + 1) ** 2, L, len(cams)) + B_rec
     traffic, traffic_src = measured_traffic(args.workload)
     out = {
         "metric": "train rays/s (fwd+bwd) at 1M Gaussians, batched vs per-view; HBM GB/s",
@@ -410,7 +432,11 @@ def main():
         "forward_kernel": {"kernel": "k_forward", "kernel_ms": fk_ms,
                            "achieved_GBps": forward_bytes(R, P) / (fk_ms * 1e-3) / 1e9},
         "step_hbm_GBps": B_step / dt_b / 1e9,
+        "step_bytes": {"total": B_step, "records": B_rec},
+        "memory": memory,
     }
+    if tune_table is not None:
+        out["autotune"] = [{"config": c, "ms": round(ms, 2)} for c, ms in tune_table]
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(wl)

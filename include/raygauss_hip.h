@@ -241,6 +241,14 @@ int rg_debug_hits(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* p
                   int32_t max_pairs, int32_t* pairs, int64_t* counts, float* rays_o, float* rays_d,
                   float* tenter, float* texit, void* stream);
 
+/* Surface-area cost of the current tree: the sum of its node boxes' surface areas over the
+ * root's, i.e. the expected number of node boxes a random ray crossing the root box visits.  A
+ * refit keeps the topology and lets the boxes grow as Gaussians move; the ratio of this cost to
+ * its value just after a rebuild tells when a rebuild pays (BatchedTrainer's rebuild policy).
+ * Synchronises the stream.  No reference counterpart: the reference rebuilds its acceleration
+ * structure on every forward (render_optix.py:107-110). */
+int rg_tree_cost(rg_handle* h, double* cost, void* stream);
+
 /* LBVH structural check (debug): returns 0 when every node box contains its children's boxes
  * and every leaf appears exactly once; nonzero counts errors. */
 int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream);

@@ -102,6 +102,7 @@ EXPORTS = {
     "rg_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats), C.c_void_p]),
     "rg_alloc_epoch": (C.c_int64, [C.c_void_p]),
     "rg_device_bytes": (C.c_int64, [C.c_void_p, C.c_void_p]),
+    "rg_tree_cost": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rg_export_predicate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "rg_debug_hits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -12,7 +12,7 @@ whenever the Gaussian count, a parameter or ray tensor, or the handle's own work
 
 from __future__ import annotations
 
-import itertools
+import dataclasses
 import json
 import os
 import time
@@ -64,10 +64,15 @@ def make_ray_batch(cams, pixels=None, device="cuda"):
 class BatchedTrainer:
     """One training pass over a fixed ray batch: refit, colour, forward + loss + backward (a
     forward launch, then a backward launch that forms the loss gradient), SH/SG backward.  With
-    use_graph=True the pass is captured in a HIP graph on first call and replayed afterwards."""
+    use_graph=True the pass is captured in a HIP graph on first call and replayed afterwards.
+
+    Rebuild policy: each step refits the tree, whose boxes grow as the Gaussians move away from
+    the positions it was built for.  Every `check_every` steps the trainer compares the tree's
+    surface-area cost (RayCaster.tree_cost) with its value just after the last rebuild and
+    rebuilds, outside the graph, once the ratio exceeds `rebuild_ratio` (None: never)."""
 
     def __init__(self, caster: RayCaster, params: dict, cams, ray_view, ray_pixel, target, loss_kind: int = 0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, rebuild_ratio: float | None = 1.5, check_every: int = 10):
         self.rc = caster
         self.p = params
         self.cams = cams
@@ -80,8 +85,23 @@ class BatchedTrainer:
         self.grads = None
         L = params["sg"].shape[2] if params.get("sg") is not None else 0
         self.L = L
+        self.rebuild_ratio = rebuild_ratio
+        self.check_every = max(1, int(check_every))
+        self.steps = 0
+        self.rebuilds = 0  # made by the policy
         self._prepare(rebuild=True)
         self.rc.set_views(cams)
+        self.base_cost = self.rc.tree_cost() if rebuild_ratio is not None else None
+        self.cost = self.base_cost
+
+    def _check_tree(self):
+        self.steps += 1
+        if self.rebuild_ratio is None or self.steps % self.check_every:
+            return
+        self.cost = self.rc.tree_cost()
+        if self.cost > self.rebuild_ratio * self.base_cost:
+            self.rebuild()
+            self.rebuilds += 1
 
     def _prepare(self, rebuild=False):
         p, L = self.p, self.L
@@ -94,14 +114,16 @@ class BatchedTrainer:
         return self.rc.finish_grads()
 
     def _key(self):
-        """Everything a captured graph depends on: sizes, tensor addresses, workspace epoch."""
+        """Everything a captured graph depends on: sizes, tensor addresses, workspace epoch and the
+        launch configuration (baked into the captured launches)."""
         ptrs = tuple((k, None if v is None else (v.data_ptr(), tuple(v.shape))) for k, v in sorted(self.p.items()))
         return (self.rc.n, ptrs, self.rv.data_ptr(), self.rp.data_ptr(), int(self.rv.shape[0]),
-                self.target.data_ptr(), self.rc.alloc_epoch())
+                self.target.data_ptr(), self.rc.alloc_epoch(), dataclasses.astuple(self.rc.tune))
 
     def step(self):
         if not self.use_graph:
             self.grads = self._pass()
+            self._check_tree()
             return self.grads
         if self.graph is not None and self.graph_key != self._key():
             self.graph = None  # stale addresses: capture again
@@ -118,6 +140,7 @@ class BatchedTrainer:
                 self.grads = self._pass()
             self.graph_key = self._key()
         self.graph.replay()
+        self._check_tree()
         return self.grads
 
     def rebuild(self, params: dict | None = None):
@@ -129,19 +152,41 @@ class BatchedTrainer:
             self.L = params["sg"].shape[2] if params.get("sg") is not None else 0
         self._prepare(rebuild=True)
         self.rc.set_views(self.cams)
+        if self.rebuild_ratio is not None:
+            self.base_cost = self.cost = self.rc.tree_cost()
         if self.graph is not None and self.graph_key != self._key():
             self.graph = None
 
 
-def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32), segs=(8, 32, 64),
-             packets=(True, False), reps: int = 2, key: str | None = None, save: bool = True):
-    """Times the train step for every (hit_depth, seg_slabs, packet) and keeps the fastest.
-    Returns (best TuneConfig, table of (config, ms))."""
-    table = []
+# Launch parameters the autotuner sweeps, with the values it tries (hit-list replay and the tree
+# builder trade memory and build time, not speed alone, and are left as configured).
+TUNE_AXES = {
+    "hit_depth": (8, 16, 32),
+    "seg_slabs": (8, 32, 64),
+    "packet": (True, False),
+    "fwd_hit_depth": (6, 12, 16),
+    "ray_chunk": (0, 64, 128),
+    "bwd_ray_chunk": (0, 64),
+}
+
+
+def autotune(caster: RayCaster, ray_view, ray_pixel, target, axes: dict | None = None, reps: int = 2,
+             passes: int = 1, key: str | None = None, save: bool = True):
+    """Coordinate search over the launch parameters of the train step (forward + L2 loss +
+    backward over the batch): starting from the caster's current TuneConfig, each axis in turn is
+    swept with the others held at the best values so far, `passes` times over all axes; each
+    configuration is run once untimed, then timed `reps` times with HIP events (the minimum
+    counts).  Returns (best TuneConfig, table of (config fields, ms)) and leaves the caster on the
+    best configuration."""
+    axes = TUNE_AXES if axes is None else axes
+    table, seen = [], {}
     loss = torch.zeros(1, dtype=torch.float32, device=ray_view.device)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for d, s, c in itertools.product(depths, segs, packets):
-        cfg = TuneConfig(d, s, packet=c)
+
+    def timed(cfg: TuneConfig) -> float:
+        k = tuple(sorted(dataclasses.asdict(cfg).items()))
+        if k in seen:
+            return seen[k]
         caster.set_tune(cfg)
         caster.train_batch(ray_view, ray_pixel, target, 0, None, loss)
         caster.zero_grads()
@@ -153,19 +198,29 @@ def autotune(caster: RayCaster, ray_view, ray_pixel, target, depths=(8, 16, 32),
             torch.cuda.synchronize()
             ms.append(e0.elapsed_time(e1))
         caster.zero_grads()
-        table.append(((d, s, c), float(min(ms))))
-    best = min(table, key=lambda r: r[1])[0]
-    cfg = TuneConfig(best[0], best[1], packet=best[2])
-    caster.set_tune(cfg)
+        seen[k] = float(min(ms))
+        table.append(({f: getattr(cfg, f) for f in axes}, seen[k]))
+        return seen[k]
+
+    best = caster.tune
+    best_ms = timed(best)
+    for _ in range(passes):
+        for field, values in axes.items():
+            for v in values:
+                cfg = dataclasses.replace(best, **{field: v})
+                ms = timed(cfg)
+                if ms < best_ms:
+                    best, best_ms = cfg, ms
+    caster.set_tune(best)
     if save and key is not None:
         try:
             data = json.load(open(TUNE_FILE)) if os.path.exists(TUNE_FILE) else {}
-            data[key] = {"hit_depth": best[0], "seg_slabs": best[1], "packet": bool(best[2]), "time": time.time()}
+            data[key] = {**dataclasses.asdict(best), "ms": best_ms, "time": time.time()}
             os.makedirs(os.path.dirname(TUNE_FILE), exist_ok=True)
             json.dump(data, open(TUNE_FILE, "w"), indent=1)
         except OSError:
             pass
-    return cfg, table
+    return best, table
 
 
 def tune_key(n: int, rays: int) -> str:
@@ -176,6 +231,6 @@ def tune_key(n: int, rays: int) -> str:
 def load_tuned(n: int, rays: int) -> TuneConfig | None:
     try:
         d = json.load(open(TUNE_FILE))[tune_key(n, rays)]
-        return TuneConfig(d["hit_depth"], d["seg_slabs"], packet=bool(d["packet"]))
+        return TuneConfig(**{f.name: d[f.name] for f in dataclasses.fields(TuneConfig) if f.name in d})
     except Exception:
         return None

@@ -331,6 +331,13 @@ class RayCaster:
         """Changes whenever the handle (re)allocates device memory (graphs over it are then stale)."""
         return int(self._L.rg_alloc_epoch(self._h))
 
+    def tree_cost(self) -> float:
+        """Surface-area cost of the current tree (sum of node box areas over the root's);
+        synchronises the stream."""
+        c = C.c_double()
+        _lib.check(self._L.rg_tree_cost(self._h, C.byref(c), _stream()), self._h)
+        return float(c.value)
+
     def device_bytes(self) -> dict:
         """Device memory the handle holds: {"total": bytes, "records": the hit-list records'
         share (rays x replay_words x 4 B + 12 B per ray)}."""

@@ -686,6 +686,25 @@ int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream) {
     return RG_OK;
 }
 
+int rg_tree_cost(rg_handle* h, double* cost, void* stream) {
+    if (!h || !cost) return fail(h, RG_ERR_ARG, "null argument");
+    if (!h->built) return fail(h, RG_ERR_STATE, "rg_prepare has not been called");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* d = reinterpret_cast<double*>(h->counters + 7);  // scratch slot (as rg_debug_check_bvh)
+    HIPCHK(h, hipMemsetAsync(d, 0, sizeof(double), s));
+    HIPCHK(h, rg::launch_tree_area((int)h->n, h->bvh, d, s));
+    float4 root[2];
+    double sum = 0.0;
+    HIPCHK(h, hipMemcpyAsync(&sum, d, sizeof(sum), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(root, h->bvh.nodes, sizeof(root), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    const double dx = (double)root[1].x - root[0].x, dy = (double)root[1].y - root[0].y,
+                 dz = (double)root[1].z - root[0].z;
+    const double a = 2.0 * (dx * dy + dy * dz + dz * dx);
+    *cost = a > 0.0 && std::isfinite(a) ? sum / a : 0.0;
+    return RG_OK;
+}
+
 int rg_adam_step(const rg_adam_tensor* tensors, int32_t count, void* stream) {
     if (count < 0 || count > rg::kMaxAdamTensors) return fail_free(RG_ERR_ARG, "rg_adam_step: 0..16 tensors");
     if (count > 0 && !tensors) return fail_free(RG_ERR_ARG, "rg_adam_step: null tensor table");

@@ -12,6 +12,8 @@
 //           flag) writes the union of its children's boxes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rg_internal.h"
 
 namespace rg {
@@ -308,6 +310,22 @@ __global__ void k_check(int n, const float4* __restrict__ nodes, const int* __re
     if (e) atomicAdd(errors, e);
 }
 
+// Sum of the node boxes' surface areas (FP64): with the root's area it gives the tree's
+// surface-area cost, the expected number of node visits of a random ray that crosses the root.
+// Boxes with a negative or non-finite extent count zero.  One FP64 atomic per wavefront.
+__global__ void k_tree_area(int total, const float4* __restrict__ nodes, double* out) {
+    double a = 0.0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const float4 lo = nodes[2 * p], hi = nodes[2 * p + 1];
+        const double dx = (double)hi.x - (double)lo.x, dy = (double)hi.y - (double)lo.y,
+                     dz = (double)hi.z - (double)lo.z;
+        const double area = 2.0 * (dx * dy + dy * dz + dz * dx);
+        if (dx >= 0.0 && dy >= 0.0 && dz >= 0.0 && area < INFINITY) a += area;
+    }
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, a);
+}
+
 inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -365,6 +383,13 @@ hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b,
     hipLaunchKernelGGL(k_fill, dim3(blocks(total, 256)), dim3(256), 0, s, b.counter, total, 0);
     hipLaunchKernelGGL(k_refit, dim3(blocks(n, 256)), dim3(256), 0, s, n, leafbox_sorted, b.nodes, b.parent, b.right,
                        b.counter, b.leaf_node);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_area(int n, const BvhBuffers& b, double* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int total = 2 * n - 1;
+    hipLaunchKernelGGL(k_tree_area, dim3(std::min(blocks(total, 256), 2048)), dim3(256), 0, s, total, b.nodes, out);
     return hipGetLastError();
 }
 

@@ -127,6 +127,8 @@ int64_t sah_task_bytes(int n);
 hipError_t launch_lbvh_finish(int n, const int* sorted_vals, BvhBuffers& b, hipStream_t s);
 hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b, hipStream_t s);
 hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s);
+// Sum of the node boxes' surface areas, FP64, added (atomically) to *out.
+hipError_t launch_tree_area(int n, const BvhBuffers& b, double* out, hipStream_t s);
 hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s);
 // Stable LSD radix sort of (keys, vals) over the low 8*passes bits; returns the index of the
 // buffer pair holding the result.  hist needs radix_hist_bytes(n).

@@ -193,9 +193,21 @@ def test_graph_replay_matches_eager(blobs, gpu_device):
 
 def test_autotune_picks_a_valid_config(blobs):
     sc, prm, cams, pixels, rc, t, rv, rp, offs = blobs
+    """The coordinate search times its starting point and then each axis's other values (one new
+    configuration per axis here), leaves the caster on the fastest, and that configuration renders
+    the same image."""
     target = torch.zeros((rv.shape[0], 3), device=rv.device)
-    cfg, table = autotune(rc, rv, rp, target, depths=(8, 16), segs=(8,), packets=(True, False), reps=1, save=False)
-    assert len(table) == 4 and cfg.hit_depth in (8, 16)
+    rc.set_tune(TuneConfig())
+    ref, _, _ = rc.forward_batch(rv, rp)
+    axes = {"hit_depth": (8, 16), "packet": (True, False), "ray_chunk": (0, 64), "bwd_ray_chunk": (0, 64)}
+    cfg, table = autotune(rc, rv, rp, target, axes=axes, reps=1, save=False)
+    assert len(table) == 5 and all(ms > 0 for _, ms in table)
+    assert len({tuple(sorted(c.items())) for c, _ in table}) == 5
+    assert rc.tune == cfg and cfg.hit_depth in (8, 16) and cfg.ray_chunk in (0, 64)
+    best = min(table, key=lambda r: r[1])[0]
+    assert all(getattr(cfg, k) == v for k, v in best.items())
+    out, _, _ = rc.forward_batch(rv, rp)
+    assert (out - ref).abs().max().item() <= 1e-5
     rc.set_tune(TuneConfig())
 
 
