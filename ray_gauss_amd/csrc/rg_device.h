@@ -288,9 +288,83 @@ __device__ __forceinline__ void traverse_packet(const float4* __restrict__ nodes
     }
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// ---------------------------------------------------------------- cross-lane operations -------
+// Wavefront reductions without LDS.  Within a row of 16 lanes, DPP moves (quad permutes, the
+// half-row mirror, the row rotation by 8) feed the ALU directly; across rows, gfx950's
+// v_permlane16_swap / v_permlane32_swap exchange row pairs and wave halves in the VALU.  (A
+// __shfl_xor compiles to ds_bpermute_b32, one LDS round trip per exchange.)  Every lane of the
+// wavefront must execute these calls.
+
+constexpr int kDppXor1 = 0xB1;         // quad_perm [1,0,3,2]: lane l reads lane l ^ 1
+constexpr int kDppXor2 = 0x4E;         // quad_perm [2,3,0,1]: lane l reads lane l ^ 2
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror: lane l reads lane l ^ 7 (groups of 8)
+constexpr int kDppRor8 = 0x128;        // row_ror:8: lane l reads lane l ^ 8 (rows of 16)
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
+// v_permlane16_swap(a, b): the two results hold, in rows 2r, a of row 2r and a of row 2r+1, and
+// in rows 2r+1, b of row 2r and b of row 2r+1 (same column).
+struct Pair {
+    unsigned int x, y;
+};
+__device__ __forceinline__ Pair swap16(unsigned int a, unsigned int b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    return Pair{r[0], r[1]};
+}
+// v_permlane32_swap(a, b): in lanes l < 32, a[l] and a[l + 32]; in lanes l >= 32, b[l - 32], b[l].
+__device__ __forceinline__ Pair swap32(unsigned int a, unsigned int b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    return Pair{r[0], r[1]};
+}
+
+// Reduce-scatter steps: rows 2r get a[l] + a[l ^ 16], rows 2r+1 get b[l ^ 16] + b[l]; and
+// lanes below 32 get a[l] + a[l ^ 32], the others b[l ^ 32] + b[l].
+__device__ __forceinline__ float pair_add16(float a, float b) {
+    const Pair r = swap16(__float_as_uint(a), __float_as_uint(b));
+    return __uint_as_float(r.x) + __uint_as_float(r.y);
+}
+__device__ __forceinline__ float pair_add32(float a, float b) {
+    const Pair r = swap32(__float_as_uint(a), __float_as_uint(b));
+    return __uint_as_float(r.x) + __uint_as_float(r.y);
+}
+
+__device__ __forceinline__ float wave_sum64(float v) {
+    v += dpp_f<kDppXor1>(v);
+    v += dpp_f<kDppXor2>(v);
+    v += dpp_f<kDppHalfMirror>(v);
+    v += dpp_f<kDppRor8>(v);
+    v = pair_add16(v, v);
+    return pair_add32(v, v);
+}
+
+__device__ __forceinline__ int wave_min64(int v) {
+    v = min(v, dpp_i<kDppXor1>(v));
+    v = min(v, dpp_i<kDppXor2>(v));
+    v = min(v, dpp_i<kDppHalfMirror>(v));
+    v = min(v, dpp_i<kDppRor8>(v));
+    Pair r = swap16((unsigned int)v, (unsigned int)v);
+    v = min((int)r.x, (int)r.y);
+    r = swap32((unsigned int)v, (unsigned int)v);
+    return min((int)r.x, (int)r.y);
+}
+
+__device__ __forceinline__ float wave_min64f(float v) {
+    v = fminf(v, dpp_f<kDppXor1>(v));
+    v = fminf(v, dpp_f<kDppXor2>(v));
+    v = fminf(v, dpp_f<kDppHalfMirror>(v));
+    v = fminf(v, dpp_f<kDppRor8>(v));
+    Pair r = swap16(__float_as_uint(v), __float_as_uint(v));
+    v = fminf(__uint_as_float(r.x), __uint_as_float(r.y));
+    r = swap32(__float_as_uint(v), __float_as_uint(v));
+    return fminf(__uint_as_float(r.x), __uint_as_float(r.y));
 }
 
 }  // namespace rg

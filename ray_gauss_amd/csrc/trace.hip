@@ -94,21 +94,6 @@ __device__ __forceinline__ float* grad_addr(const DevGrad& gr, int n, int slot, 
     return reinterpret_cast<float*>(gr.drgb + (size_t)view * n + slot) + (k - 11);
 }
 
-__device__ __forceinline__ float wave_sum64(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ int wave_min64(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ float wave_min64f(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 __device__ __forceinline__ void load_rec(const float4* __restrict__ geo, int s, float4& g0, float4& g1, float4& g2,
                                          float4& g3) {
     g0 = geo[4 * s];
@@ -340,36 +325,35 @@ __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScen
     __syncthreads();
 }
 
-// Butterfly step of the reduce-scatter below: the lanes with `upper` keep the upper half of
-// their H values and receive the partner's (lane ^ O) upper half; the others keep the lower.
-template <int H, int O>
-__device__ __forceinline__ void rs_step(const float* in, float* out, bool upper) {
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-        const float keep = upper ? in[j + H] : in[j];
-        const float send = upper ? in[j] : in[j + H];
-        out[j] = keep + __shfl_xor(send, O, 64);
-    }
-}
-
 // Wave-wide sum of the 14 gradient values of one Gaussian (all lanes hold a contribution, zero
 // for lanes without one), appended to the staging area as record nst (flushed when full).  The
-// sums are formed by a reduce-scatter: after steps over lane bits 5..2 lane l holds value
-// bitrev4(l >> 2) summed over 16 lanes, and two more steps over bits 1..0 complete it (17
-// exchanges instead of 14 x 6).  Called by all 64 lanes together.
+// sums are formed by a reduce-scatter: the lanes below 32 keep values 0..7 and add the upper
+// lanes' copies of them, the upper lanes values 8..15 (v_permlane32_swap); then the same over
+// row pairs (v_permlane16_swap), over lane bit 3 (DPP row rotation by 8) and bit 2 (DPP
+// half-row mirror: the partner differs in bits 2..0, and the quad sum below covers bits 1..0);
+// after that lane l holds value bitrev4(l >> 2) summed over its quad's partners, and two DPP
+// steps complete the sum over the quad.  No LDS traffic.  Called by all 64 lanes together.
 __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int view, const DevScene& sc,
                                                   const DevGrad& gr, int lane, GradStage& G, int& nst,
                                                   unsigned long long* drops) {
     float a[16], b[8], c[4], d[2];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = k < kGradVals ? v[k] : 0.0f;
-    rs_step<8, 32>(a, b, (lane & 32) != 0);
-    rs_step<4, 16>(b, c, (lane & 16) != 0);
-    rs_step<2, 8>(c, d, (lane & 8) != 0);
-    float x = ((lane & 4) != 0) ? d[1] : d[0];
-    x += __shfl_xor(((lane & 4) != 0) ? d[0] : d[1], 4, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = pair_add32(a[j], a[j + 8]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = pair_add16(b[j], b[j + 4]);
+    const bool up8 = (lane & 8) != 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float keep = up8 ? c[j + 2] : c[j];
+        const float send = up8 ? c[j] : c[j + 2];
+        d[j] = keep + dpp_f<kDppRor8>(send);
+    }
+    const bool up4 = (lane & 4) != 0;
+    float x = (up4 ? d[1] : d[0]) + dpp_f<kDppHalfMirror>(up4 ? d[0] : d[1]);
+    x += dpp_f<kDppXor2>(x);
+    x += dpp_f<kDppXor1>(x);
     const int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
     if ((lane & 3) == 0) G.val[nst][idx] = x;
     if (lane == 0) {
