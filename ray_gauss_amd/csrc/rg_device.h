@@ -155,9 +155,12 @@ __device__ __forceinline__ float isect_terms(const float4 g0, const float4 g1, c
 __device__ __forceinline__ bool intersect(const float4 g0, const float4 g1, const float4 g2, const float4 g3,
                                           const Ray& r, Isect& I) {
 #pragma clang fp contract(off)
-    if (!(g1.w > 0.0f)) return false;  // h <= 0 or NaN: no bounding ellipsoid (|sigma| <= thr)
+    // The terms are computed before the validity test (h <= 0 or NaN: no bounding ellipsoid,
+    // |sigma| <= thr; the terms of such a record are then unused), so that the record's four
+    // loads are issued together instead of waiting for h before loading the rest.
     float b, c;
     const float disc = isect_terms(g0, g1, g2, g3, r, I, b, c);
+    if (!(g1.w > 0.0f)) return false;
     if (!(disc > 1e-7f)) return false;
     const float sd = __builtin_sqrtf(disc);
     const float q = (b > 0.0f) ? b + sd : b - sd;
