@@ -62,12 +62,22 @@ struct IList {
 // Bit e set for the list entries (e < cnt) whose interval meets the slab [tmin, tmax]: the
 // lanes then loop over their own hits only (in increasing entry = slot order), so a wave
 // iterates max-over-lanes(hits) times instead of max-over-lanes(cnt).
+//
+// Every listed interval met its gathered segment, so r1 <= r2 (or one of them is NaN), and every
+// slab has tmin <= tmax; for such operands overlaps(tmin, tmax, r1, r2) is !(r1 > tmax) and
+// !(r2 < tmin), NaN ends included (overlaps() lets fmaxf / fminf drop a NaN).  That form needs
+// no NaN-quieting max/min and no branch per entry: the loop ends at the longest list among the
+// lanes executing the call (a ballot, which counts only those lanes), and each lane masks off the
+// entries past its own count.
 template <int CAP>
 __device__ __forceinline__ unsigned int slab_mask(const IList<CAP>& L, int cnt, float tmin, float tmax, int lane) {
     unsigned int m = 0u;
 #pragma unroll
-    for (int e = 0; e < CAP; ++e)
-        if (e < cnt && overlaps(tmin, tmax, L.r1[e][lane], L.r2[e][lane])) m |= 1u << e;
+    for (int e = 0; e < CAP; ++e) {
+        if (__builtin_amdgcn_ballot_w64(e < cnt) == 0ULL) break;
+        const bool in = (e < cnt) & !(L.r1[e][lane] > tmax) & !(L.r2[e][lane] < tmin);
+        m |= in ? 1u << e : 0u;
+    }
     return m;
 }
 
@@ -889,6 +899,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         int cur_cnt = 0;
         bool cur_dlist = false;  // the entries are a direct slab's recorded hits (all in the slab)
         int cur_dpos = 0;
+        unsigned int cur_m = 0u;  // list mode: the entries in the slab (slab_mask), not yet scattered
         // the prepared slab's sample buffers; after slab_coeffs sig, cr, cg hold the per-sample
         // backward coefficients A_i, Bq_i, E_i read by the scatter below
         float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
@@ -934,6 +945,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     s.dense = nh > CAP;
                 } else {
                     unsigned int m = slab_mask<CAP>(L, s.cnt, tmin, tmax, lane);
+                    cur_m = m;
                     while (m != 0u) {
                         const int e = __builtin_ctz(m);
                         m &= m - 1u;
@@ -983,14 +995,9 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     }
                     ++e;  // an out-of-range slot was counted when the slab was prepared
                 }
-            } else if (has) {
-                while (e < cur_cnt) {
-                    if (overlaps(cur_tmin, cur_tmax, L.r1[e][lane], L.r2[e][lane])) {
-                        my = key_base + L.g[e][lane];
-                        break;
-                    }
-                    ++e;
-                }
+            } else if (has && cur_m != 0u) {
+                e = __builtin_ctz(cur_m);  // the slab's next entry (entries are in slot order)
+                my = key_base + L.g[e][lane];
             }
             const int mn = wave_min64(my);
             if (mn == INT_MAX) break;
@@ -1020,6 +1027,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                                         v);
                 }
                 ++e;
+                cur_m &= cur_m - 1u;
             }
             const int mview = mn / sc.n, mslot = mn - mview * sc.n;
             grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst, drops);
