@@ -227,14 +227,21 @@ __device__ __forceinline__ void slab_coeffs(float* sig, float* cr, float* cg, fl
     }
 }
 
-// Gradient of one (slab, hit): four sufficient statistics over the slab's samples, expanded
-// into v = (d mu[3], d sigma, d s[3], d q[4] (w,x,y,z), d rgb[3]).
+// Gradient moments of one (slab, hit) for one ray.  Over the slab's samples the hit contributes
+// four sums (Ss = sum w phi, S1 = sum w phi tau, S2 = sum w phi tau^2, Sc = sum w Bq; tau = t - t_c).
+// Every gradient of the Gaussian is linear in the ray-dependent moments
+//   Gy'   = P S0 + D T1                                     (3)
+//   Y'    = P P^T S0 + (P D^T + D P^T) T1 + D D^T T2         (6, symmetric)
+//   Ss, and dL_c sigma Sc                                   (1 + 3)
+// (S0, T1, T2 = sigma Ss, sigma S1, sigma S2; P, D the ray's ellipsoid-space offset and direction)
+// with coefficients that depend on the Gaussian alone, so the wave sums these 13 moments over its
+// lanes and expand the sum once per Gaussian (expand_grad) instead of once per lane.
+constexpr int kMoments = 13;
+
 template <int SPB, bool RNG>
-__device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, float4 g1, float4 g2, float4 g3,
-                                          float4 col, float tb, unsigned int seed, const DevParams& p, const float* Aq,
-                                          const float* Bq, const float* Eq, const float* dL, const DevScene& sc,
-                                          float* v) {
-    const float sigma = g0.w;
+__device__ __forceinline__ void grad_moments(const Isect& I, float sigma, float4 col, float tb, unsigned int seed,
+                                             const DevParams& p, const float* Aq, const float* Bq, const float* Eq,
+                                             const float* dL, float* v) {
     const float F = dL[0] * col.x + dL[1] * col.y + dL[2] * col.z;
     float Ss = 0.0f, S1 = 0.0f, S2 = 0.0f, Sc = 0.0f;
     unsigned int st = seed;
@@ -253,20 +260,36 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
         S2 += wp * (tau * tau);
         Sc += w * Bq[i];
     }
-    const float4 b0 = sc.brec[2 * slot], q = sc.brec[2 * slot + 1];
-    const float d = b0.w;
     const float S0 = sigma * Ss, T1 = sigma * S1, T2 = sigma * S2;
-    // y(t) = yc + tau v in the unenlarged ellipsoid frame
-    const float yc[3] = {d * I.Px, d * I.Py, d * I.Pz};
-    const float vv[3] = {d * I.Dx, d * I.Dy, d * I.Dz};
-    float Gy[3], Y[3][3];
+    const float P[3] = {I.Px, I.Py, I.Pz};
+    const float D[3] = {I.Dx, I.Dy, I.Dz};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        Gy[a] = yc[a] * S0 + vv[a] * T1;
+    for (int a = 0; a < 3; ++a) v[a] = P[a] * S0 + D[a] * T1;
+    // Y' entries (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+    const int ya[6] = {0, 1, 2, 0, 0, 1}, yb[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
-            Y[a][b] = yc[a] * yc[b] * S0 + (yc[a] * vv[b] + vv[a] * yc[b]) * T1 + vv[a] * vv[b] * T2;
+    for (int e = 0; e < 6; ++e) {
+        const int a = ya[e], b = yb[e];
+        v[3 + e] = P[a] * P[b] * S0 + (P[a] * D[b] + D[a] * P[b]) * T1 + D[a] * D[b] * T2;
     }
+    v[9] = Ss;
+    const float sc_ = sigma * Sc;
+    v[10] = dL[0] * sc_;
+    v[11] = dL[1] * sc_;
+    v[12] = dL[2] * sc_;
+}
+
+// The 14 gradient values of one Gaussian from its summed moments m (grad_moments):
+// v = (d mu[3], d sigma, d s[3], d q[4] (w,x,y,z), d rgb[3]), with the Gaussian's record rows of
+// M' (g1..g3), scales s and delta (brec[0]), quaternion (brec[1]); y = d (P + tau D) in the
+// unenlarged ellipsoid frame, so Gy = d Gy', Y = d^2 Y'.
+__device__ __forceinline__ void expand_grad(const float* mo, float4 g1, float4 g2, float4 g3, float4 b0, float4 q,
+                                            float* v) {
+    const float d = b0.w, d2 = d * d;
+    const float Gy[3] = {d * mo[0], d * mo[1], d * mo[2]};
+    const float Y[3][3] = {{d2 * mo[3], d2 * mo[6], d2 * mo[7]},
+                           {d2 * mo[6], d2 * mo[4], d2 * mo[8]},
+                           {d2 * mo[7], d2 * mo[8], d2 * mo[5]}};
     const float m[3][3] = {{g1.x, g1.y, g1.z}, {g2.x, g2.y, g2.z}, {g3.x, g3.y, g3.z}};  // rows of M'
     const float s[3] = {b0.x, b0.y, b0.z};
     float dcol[3][3];
@@ -287,7 +310,7 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
     const float* c0 = dcol[0];
     const float* c1 = dcol[1];
     const float* c2 = dcol[2];
-    v[3] = Ss;
+    v[3] = mo[9];
     v[4] = Y[0][0] * is[0];
     v[5] = Y[1][1] * is[1];
     v[6] = Y[2][2] * is[2];
@@ -298,10 +321,9 @@ __device__ __forceinline__ void grad_vals(const Isect& I, int slot, float4 g0, f
            4.0f * qj * (c0[0] + c2[2]);
     v[10] = 2.0f * (r * c0[1] + qi * c0[2] - r * c1[0] + qj * c1[2] + qi * c2[0] + qj * c2[1]) -
             4.0f * qk * (c0[0] + c1[1]);
-    const float sc_ = sigma * Sc;
-    v[11] = dL[0] * sc_;
-    v[12] = dL[1] * sc_;
-    v[13] = dL[2] * sc_;
+    v[11] = mo[10];
+    v[12] = mo[11];
+    v[13] = mo[12];
 }
 
 // Staging area of the wave-merged gradient sums in LDS.  Global atomics are issued in bursts of
@@ -321,6 +343,20 @@ struct GradStage {
 __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScene& sc, const DevGrad& gr,
                                             int lane, unsigned long long* drops) {
     __syncthreads();
+    // lane e expands record e's summed moments into its 14 gradient values, in place
+    if (lane < cnt) {
+        const int slot = G.slot[lane];
+        if ((unsigned)slot < (unsigned)sc.n) {
+            float mo[kMoments], v[kGradVals];
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k) mo[k] = G.val[lane][k];
+            const float4 g1 = sc.geo[4 * slot + 1], g2 = sc.geo[4 * slot + 2], g3 = sc.geo[4 * slot + 3];
+            expand_grad(mo, g1, g2, g3, sc.brec[2 * slot], sc.brec[2 * slot + 1], v);
+#pragma unroll
+            for (int k = 0; k < kGradVals; ++k) G.val[lane][k] = v[k];
+        }
+    }
+    __syncthreads();
     const int j = lane & 15;
     for (int e0 = 0; e0 < cnt; e0 += 4) {
         const int e = e0 + (lane >> 4);
@@ -335,8 +371,9 @@ __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScen
     __syncthreads();
 }
 
-// Wave-wide sum of the 14 gradient values of one Gaussian (all lanes hold a contribution, zero
-// for lanes without one), appended to the staging area as record nst (flushed when full).  The
+// Wave-wide sum of the 13 gradient moments of one Gaussian (all lanes hold a contribution, zero
+// for lanes without one), appended to the staging area as record nst (expanded and flushed when
+// full).  The
 // sums are formed by a reduce-scatter: the lanes below 32 keep values 0..7 and add the upper
 // lanes' copies of them, the upper lanes values 8..15 (v_permlane32_swap); then the same over
 // row pairs (v_permlane16_swap), over lane bit 3 (DPP row rotation by 8) and bit 2 (DPP
@@ -348,7 +385,7 @@ __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int 
                                                   unsigned long long* drops) {
     float a[16], b[8], c[4], d[2];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) a[k] = k < kGradVals ? v[k] : 0.0f;
+    for (int k = 0; k < 16; ++k) a[k] = k < kMoments ? v[k] : 0.0f;
 #pragma unroll
     for (int j = 0; j < 8; ++j) b[j] = pair_add32(a[j], a[j + 8]);
 #pragma unroll
@@ -841,9 +878,9 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             continue;
         }
         if (leaf >= sc.n) break;  // malformed leaf index (uniform)
-        float v[kGradVals];
+        float v[kMoments];
 #pragma unroll
-        for (int k = 0; k < kGradVals; ++k) v[k] = 0.0f;
+        for (int k = 0; k < kMoments; ++k) v[k] = 0.0f;
         bool c = false;
         if (h) {
             float4 g0, g1, g2, g3;
@@ -851,7 +888,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             Isect I;
             if (intersect(g0, g1, g2, g3, ray, I) && overlaps(cur_tmin, cur_tmax, I.r1, I.r2)) {
                 refine_sampling(g0, g1, g2, g3, ray, I);
-                grad_vals<SPB, RNG>(I, leaf, g0, g1, g2, g3, rgbv[leaf], cur_tb, cur_seed, p, Aq, Bq, Eq, dL, sc, v);
+                grad_moments<SPB, RNG>(I, g0.w, rgbv[leaf], cur_tb, cur_seed, p, Aq, Bq, Eq, dL, v);
                 c = true;
             }
         }
@@ -859,9 +896,9 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
         while (rem != 0ULL) {  // one record per view among the contributing lanes (normally one)
             const int vw = __builtin_amdgcn_readlane(view, __ffsll((long long)rem) - 1);
             const bool mine = c && view == vw;
-            float w[kGradVals];
+            float w[kMoments];
 #pragma unroll
-            for (int k = 0; k < kGradVals; ++k) w[k] = mine ? v[k] : 0.0f;
+            for (int k = 0; k < kMoments; ++k) w[k] = mine ? v[k] : 0.0f;
             grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst, drops);
             rem &= ~__ballot(mine);
         }
@@ -1002,9 +1039,9 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
             const int mn = wave_min64(my);
             if (mn == INT_MAX) break;
             const bool act = (my == mn);
-            float v[kGradVals];
+            float v[kMoments];
 #pragma unroll
-            for (int k = 0; k < kGradVals; ++k) v[k] = 0.0f;
+            for (int k = 0; k < kMoments; ++k) v[k] = 0.0f;
             if (act) {
                 const int slot = mn - key_base;
                 float4 g0, g1, g2, g3;
@@ -1023,8 +1060,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 }
                 if (ok) {
                     refine_sampling(g0, g1, g2, g3, ray, I);
-                    grad_vals<SPB, RNG>(I, slot, g0, g1, g2, g3, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, sc,
-                                        v);
+                    grad_moments<SPB, RNG>(I, g0.w, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, v);
                 }
                 ++e;
                 cur_m &= cur_m - 1u;
