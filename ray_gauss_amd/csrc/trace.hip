@@ -970,11 +970,20 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     ++nh;
                     return true;
                 };
-                if (s.direct && RM == 2 && s.dcnt >= 0) {  // the forward's hits of this slab
+                if (s.direct && RM == 2 && s.dcnt >= 0) {
+                    // the forward's hits of this slab: they passed the exact test in the forward over
+                    // this same ray (the token, the handle's generation and the row's (view, pixel)
+                    // are checked), so only their sampling terms are computed, as for list entries
                     for (int i = 0; i < s.dcnt; ++i) {
                         const int slot = (int)*rec_at(rc, s.dpos + i);
-                        if ((unsigned)slot < (unsigned)sc.n) direct_hit(slot);
-                        else count_drop(drops);
+                        if ((unsigned)slot < (unsigned)sc.n) {
+                            float sigma;
+                            const Isect I = entry_hit(sc, ray, slot, 0.0f, 0.0f, sigma);
+                            accum_hit<SPB, RNG>(I, sigma, rgbv[slot], tb, seed, p, sig, cr, cg, cb);
+                            ++nh;
+                        } else {
+                            count_drop(drops);
+                        }
                     }
                     s.dense = nh > CAP;
                 } else if (s.direct) {
@@ -1047,18 +1056,12 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 float4 g0, g1, g2, g3;
                 load_rec(sc.geo, slot, g0, g1, g2, g3);
                 Isect I;
-                // a recorded direct hit reproduces the forward's test (a record that does not is
-                // skipped rather than scattered); a list entry passed it when it was gathered, so
-                // only its sampling terms are computed (the same values, without the roots)
-                bool ok;
-                if (cur_dlist) {
-                    ok = intersect(g0, g1, g2, g3, ray, I) && overlaps(cur_tmin, cur_tmax, I.r1, I.r2);
-                } else {
+                // a list entry or a recorded direct hit passed the exact test when it was gathered
+                // or recorded, so only its sampling terms are computed (the same values, without the
+                // roots)
+                {
                     float b, c;
                     isect_terms(g0, g1, g2, g3, ray, I, b, c);
-                    ok = true;
-                }
-                if (ok) {
                     refine_sampling(g0, g1, g2, g3, ray, I);
                     grad_moments<SPB, RNG>(I, g0.w, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, v);
                 }
