@@ -74,6 +74,9 @@ typedef struct rg_config {
     int32_t refill_lanes;      /* persistent forward: 0 = a wavefront takes new rays when all its
                                   lanes are done; 1..63 = as soon as that many lanes are idle   */
     int32_t bwd_ray_chunk;     /* launch structure of the backward kernel, as ray_chunk         */
+    float bwd_window;          /* backward: lanes within this many slabs of the wave's laggard
+                                  prepare their next slab together and merge their gradients
+                                  (<= 0: the default, 0.5; the best value depends on the scene) */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

@@ -167,6 +167,7 @@ TUNE_AXES = {
     "fwd_hit_depth": (6, 12, 16),
     "ray_chunk": (0, 64, 128),
     "bwd_ray_chunk": (0, 64),
+    "bwd_window": (0.5, 1.0, 2.0),
 }
 
 

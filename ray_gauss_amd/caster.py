@@ -32,7 +32,10 @@ class TuneConfig:
     and stealing from the others' when their own is empty (64: 3 % faster at 1M Gaussians);
     refill_lanes: in that mode the forward hands new rays to idle lanes once that many are idle
     (0: only when all its lanes are done; refilling was measured 2.5x slower, see the kernel);
-    bwd_ray_chunk: the same for the backward (persistent measured 9 % slower at 1M Gaussians)."""
+    bwd_ray_chunk: the same for the backward (persistent measured 9 % slower at 1M Gaussians);
+    bwd_window: the backward's depth synchronisation, in slabs: lanes within this distance of the
+    wave's laggard prepare their next slab together and merge their gradients (0.5 is best at 1M
+    Gaussians, 1.0 on the 300k-Gaussian batch of uneven views)."""
 
     hit_depth: int = 16
     seg_slabs: int = 32
@@ -43,6 +46,7 @@ class TuneConfig:
     ray_chunk: int = 64
     refill_lanes: int = 0
     bwd_ray_chunk: int = 0
+    bwd_window: float = 0.5
 
 
 def _ptr(t):
@@ -127,6 +131,7 @@ class RayCaster:
         c.ray_chunk = int(getattr(self.tune, "ray_chunk", 0))
         c.refill_lanes = int(getattr(self.tune, "refill_lanes", 0))
         c.bwd_ray_chunk = int(getattr(self.tune, "bwd_ray_chunk", 0))
+        c.bwd_window = float(getattr(self.tune, "bwd_window", 0.5))
         c.jitter = int(getattr(p, "jitter", 0))
         c.rnd_sample = int(getattr(p, "rnd_sample", 0))
         c.iteration = int(getattr(p, "iteration", 0) or 0) & 0xFFFFFFFF

@@ -142,6 +142,7 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->refill_lanes < 0 || c->refill_lanes > 64) return fail(h, RG_ERR_ARG, "refill_lanes must be in 0..64");
     if (c->bwd_ray_chunk != 0 && (c->bwd_ray_chunk < 64 || c->bwd_ray_chunk > 65536 || (c->bwd_ray_chunk & 63)))
         return fail(h, RG_ERR_ARG, "bwd_ray_chunk must be 0 or a multiple of 64 in 64..65536");
+    if (!(c->bwd_window <= 1024.0f)) return fail(h, RG_ERR_ARG, "bwd_window must be at most 1024 slabs");
     return RG_OK;
 }
 
@@ -236,6 +237,7 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.ray_chunk = h->cfg.ray_chunk;
     p.refill_lanes = h->cfg.refill_lanes;
     p.queue = h->queue;
+    p.bwd_window = h->cfg.bwd_window > 0.0f ? h->cfg.bwd_window : 0.5f;
     return p;
 }
 

@@ -66,6 +66,8 @@ struct DevParams {
     int ray_chunk;
     int refill_lanes;
     unsigned long long* queue;
+    // backward depth synchronisation, in slabs (lanes within it of the wave's laggard step together)
+    float bwd_window;
 };
 
 // Gradient accumulators (device-scope atomic adds; the finish kernel reads and clears them).

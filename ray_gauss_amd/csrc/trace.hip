@@ -923,9 +923,9 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
     SegState s;
     seg_init(s, ray, p);
     const int key_base = view * sc.n;
-    // lanes within half a slab of the laggard step together (measured: half a slab 6 % faster
-    // than one, two slabs 6 % slower; a quarter no different)
-    const float window = 0.5f * p.dt * (float)p.spb;
+    // lanes within p.bwd_window slabs of the laggard step together (half a slab at 1M Gaussians:
+    // 4 % faster than one, the same as a quarter; one slab 12 % faster on a 300k-Gaussian batch)
+    const float window = p.bwd_window * p.dt * (float)p.spb;
     bool pend = false;  // next_slab() has produced a slab not yet processed
     float tmin = 0.0f, tmax = 0.0f;
     while (__ballot(alive) != 0ULL) {
