@@ -333,6 +333,7 @@ __device__ __forceinline__ void expand_grad(const float* mo, float4 g1, float4 g
 constexpr int kStage = 16;
 struct GradStage {
     float val[kStage][16];
+    int key[kStage];  // view * n + slot, decoded at the flush
     int slot[kStage];
     int view[kStage];
 };
@@ -343,9 +344,14 @@ struct GradStage {
 __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScene& sc, const DevGrad& gr,
                                             int lane, unsigned long long* drops) {
     __syncthreads();
-    // lane e expands record e's summed moments into its 14 gradient values, in place
+    // lane e decodes record e's key and expands its summed moments into its 14 gradient values,
+    // in place
     if (lane < cnt) {
-        const int slot = G.slot[lane];
+        const int key = G.key[lane];
+        const int view = key / sc.n;
+        const int slot = key - view * sc.n;
+        G.slot[lane] = slot;
+        G.view[lane] = view;
         if ((unsigned)slot < (unsigned)sc.n) {
             float mo[kMoments], v[kGradVals];
 #pragma unroll
@@ -372,15 +378,15 @@ __device__ __forceinline__ void stage_flush(GradStage& G, int cnt, const DevScen
 }
 
 // Wave-wide sum of the 13 gradient moments of one Gaussian (all lanes hold a contribution, zero
-// for lanes without one), appended to the staging area as record nst (expanded and flushed when
-// full).  The
-// sums are formed by a reduce-scatter: the lanes below 32 keep values 0..7 and add the upper
-// lanes' copies of them, the upper lanes values 8..15 (v_permlane32_swap); then the same over
-// row pairs (v_permlane16_swap), over lane bit 3 (DPP row rotation by 8) and bit 2 (DPP
-// half-row mirror: the partner differs in bits 2..0, and the quad sum below covers bits 1..0);
-// after that lane l holds value bitrev4(l >> 2) summed over its quad's partners, and two DPP
-// steps complete the sum over the quad.  No LDS traffic.  Called by all 64 lanes together.
-__device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int view, const DevScene& sc,
+// for lanes without one), appended to the staging area as record nst under its key view * n + slot
+// (decoded, expanded and flushed when the area is full).  The sums are formed by a
+// reduce-scatter: the lanes below 32 keep values 0..7 and add the upper lanes' copies of them,
+// the upper lanes values 8..15 (v_permlane32_swap); then the same over row pairs
+// (v_permlane16_swap), over lane bit 3 (DPP row rotation by 8) and bit 2 (DPP half-row mirror:
+// the partner differs in bits 2..0, and the quad sum below covers bits 1..0); after that lane l
+// holds value bitrev4(l >> 2) summed over its quad's partners, and two DPP steps complete the sum
+// over the quad.  No LDS traffic.  Called by all 64 lanes together.
+__device__ __forceinline__ void grad_scatter_wave(const float* v, int key, const DevScene& sc,
                                                   const DevGrad& gr, int lane, GradStage& G, int& nst,
                                                   unsigned long long* drops) {
     float a[16], b[8], c[4], d[2];
@@ -403,10 +409,7 @@ __device__ __forceinline__ void grad_scatter_wave(const float* v, int slot, int 
     x += dpp_f<kDppXor1>(x);
     const int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
     if ((lane & 3) == 0) G.val[nst][idx] = x;
-    if (lane == 0) {
-        G.slot[nst] = slot;
-        G.view[nst] = view;
-    }
+    if (lane == 0) G.key[nst] = key;
     if (++nst == kStage) {
         stage_flush(G, kStage, sc, gr, lane, drops);
         nst = 0;
@@ -899,7 +902,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             float w[kMoments];
 #pragma unroll
             for (int k = 0; k < kMoments; ++k) w[k] = mine ? v[k] : 0.0f;
-            grad_scatter_wave(w, leaf, vw, sc, gr, lane, G, nst, drops);
+            grad_scatter_wave(w, vw * sc.n + leaf, sc, gr, lane, G, nst, drops);
             rem &= ~__ballot(mine);
         }
         idx = miss;
@@ -1028,23 +1031,22 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         if (__ballot(dsc) != 0ULL)
             direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, cg, dL, G,
                                      nst, lane, gr, view, drops);
-        // wave-merged scatter over the prepared slabs' list hits
+        // wave-merged scatter over the prepared slabs' list hits.  A lane's key (view * n + slot of
+        // its next hit; INT_MAX when it has none) changes only when its hit is scattered, and the
+        // following hit's slot is loaded ahead of that hit's arithmetic.
         int e = 0;
-        while (true) {
-            int my = INT_MAX;
-            if (has && cur_dlist) {
-                while (e < cur_cnt) {
-                    const int sl = (int)*rec_at(rc, cur_dpos + e);
-                    if ((unsigned)sl < (unsigned)sc.n) {
-                        my = key_base + sl;
-                        break;
-                    }
-                    ++e;  // an out-of-range slot was counted when the slab was prepared
-                }
-            } else if (has && cur_m != 0u) {
-                e = __builtin_ctz(cur_m);  // the slab's next entry (entries are in slot order)
-                my = key_base + L.g[e][lane];
+        auto dlist_key = [&]() -> int {
+            while (e < cur_cnt) {
+                const int sl = (int)*rec_at(rc, cur_dpos + e);
+                if ((unsigned)sl < (unsigned)sc.n) return key_base + sl;
+                ++e;  // an out-of-range slot was counted when the slab was prepared
             }
+            return INT_MAX;
+        };
+        int my = INT_MAX;
+        if (has && cur_dlist) my = dlist_key();
+        else if (has && cur_m != 0u) my = key_base + L.g[__builtin_ctz(cur_m)][lane];  // slot order
+        while (true) {
             const int mn = wave_min64(my);
             if (mn == INT_MAX) break;
             const bool act = (my == mn);
@@ -1053,6 +1055,13 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
             for (int k = 0; k < kMoments; ++k) v[k] = 0.0f;
             if (act) {
                 const int slot = mn - key_base;
+                int nxt = -1;
+                if (cur_dlist) {
+                    if (e + 1 < cur_cnt) nxt = (int)*rec_at(rc, cur_dpos + e + 1);
+                } else {
+                    cur_m &= cur_m - 1u;
+                    if (cur_m != 0u) nxt = L.g[__builtin_ctz(cur_m)][lane];
+                }
                 float4 g0, g1, g2, g3;
                 load_rec(sc.geo, slot, g0, g1, g2, g3);
                 Isect I;
@@ -1065,11 +1074,14 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     refine_sampling(g0, g1, g2, g3, ray, I);
                     grad_moments<SPB, RNG>(I, g0.w, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, v);
                 }
-                ++e;
-                cur_m &= cur_m - 1u;
+                if (cur_dlist) {
+                    ++e;
+                    my = (unsigned)nxt < (unsigned)sc.n ? key_base + nxt : dlist_key();
+                } else {
+                    my = cur_m != 0u ? key_base + nxt : INT_MAX;
+                }
             }
-            const int mview = mn / sc.n, mslot = mn - mview * sc.n;
-            grad_scatter_wave(v, mslot, mview, sc, gr, lane, G, nst, drops);
+            grad_scatter_wave(v, mn, sc, gr, lane, G, nst, drops);
         }
     }
     stage_flush(G, nst, sc, gr, lane, drops);
