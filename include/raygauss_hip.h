@@ -218,9 +218,10 @@ int rg_scene_box(rg_handle* h, float* out6_device, void* stream);
 int rg_get_stats(rg_handle* h, rg_stats* out, void* stream);
 
 /* Allocation epoch of the handle's device workspace: it changes whenever a call (re)allocates a
- * buffer (a larger n, more views or rays, a first recording forward).  A HIP graph captured
- * over this handle's calls bakes in those addresses and stays valid only while the epoch is
- * unchanged.  No reference counterpart (the reference captures no graphs). */
+ * buffer (a larger n, more views or rays, a first recording forward) and at every binned-SAH
+ * rebuild (the refit launches one kernel per level of that tree).  A HIP graph captured over
+ * this handle's calls bakes in those addresses and launches and stays valid only while the
+ * epoch is unchanged.  No reference counterpart (the reference captures no graphs). */
 int64_t rg_alloc_epoch(const rg_handle* h);
 
 /* Device memory the handle holds: the sum of its live allocations (the tree, per-Gaussian
@@ -252,8 +253,9 @@ int rg_debug_hits(rg_handle* h, int32_t view, int64_t num_rays, const int32_t* p
  * structure on every forward (render_optix.py:107-110). */
 int rg_tree_cost(rg_handle* h, double* cost, void* stream);
 
-/* LBVH structural check (debug): returns 0 when every node box contains its children's boxes
- * and every leaf appears exactly once; nonzero counts errors. */
+/* Tree check (debug, either builder): *errors = 0 when the links are consistent, every leaf
+ * appears exactly once with its Gaussian's box, and every internal box is exactly (bit for bit)
+ * the union of its children's boxes, as the refit computes it; otherwise the number of faults. */
 int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream);
 
 /* ---- optimiser step (handle-free; errors via rg_last_error(NULL)) ------------------------- */

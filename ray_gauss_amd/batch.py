@@ -146,7 +146,9 @@ class BatchedTrainer:
     def rebuild(self, params: dict | None = None):
         """Rebuild the tree outside the graph, optionally over new parameter tensors (after
         densification).  With the same Gaussian count and tensors the handle's buffers keep their
-        addresses and the graph stays valid; otherwise the next step captures again."""
+        addresses and an LBVH's graph stays valid; a binned-SAH rebuild (whose refit runs one
+        launch per tree level) or new sizes or tensors change the key, and the next step captures
+        again."""
         if params is not None:
             self.p = params
             self.L = params["sg"].shape[2] if params.get("sg") is not None else 0

@@ -180,6 +180,7 @@ int ensure_gaussian_capacity(rg_handle* h, int64_t n) {
     if ((rc = dalloc(h, &b.sah_tasks[0], task_f4))) return rc;
     if ((rc = dalloc(h, &b.sah_tasks[1], task_f4))) return rc;
     if ((rc = dalloc(h, &b.sah_ctr, 4))) return rc;
+    if ((rc = dalloc(h, &b.k_pre, cap))) return rc;
     h->cap = cap;
     h->built = false;
     return RG_OK;
@@ -318,7 +319,7 @@ int rg_destroy(rg_handle* h) {
                     h->bvh.parent, h->bvh.right, h->bvh.counter, h->bvh.leaf_node, h->bvh.k_parent,
                     h->bvh.k_left, h->bvh.k_right, h->bvh.k_first, h->bvh.k_last, h->cams, h->rgb, h->drgb,
                     h->train_rgb, h->rec, h->rec_len, h->rec_id, h->bvh.sah_prims[0], h->bvh.sah_prims[1],
-                    h->bvh.sah_tasks[0], h->bvh.sah_tasks[1], h->bvh.sah_ctr};
+                    h->bvh.sah_tasks[0], h->bvh.sah_tasks[1], h->bvh.sah_ctr, h->bvh.k_pre};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete h;
@@ -372,6 +373,8 @@ int rg_prepare(rg_handle* h, const rg_gaussians* g, int32_t rebuild, void* strea
             const hipError_t e = rg::launch_sah_build(n, h->leafbox, h->geo, h->box, h->bvh, s);
             if (e == hipErrorUnknown) return fail(h, RG_ERR_HIP, "SAH build: inconsistent partition (internal error)");
             HIPCHK(h, e);
+            // the refit's launches follow this tree's levels: a graph captured before is stale
+            h->alloc_epoch++;
         } else {
             HIPCHK(h, rg::launch_lbvh_build(n, h->leafbox, h->box, h->bvh, s));
         }
@@ -680,7 +683,7 @@ int rg_debug_check_bvh(rg_handle* h, int64_t* errors, void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     unsigned long long* d = h->counters + 7;
     HIPCHK(h, hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
-    HIPCHK(h, rg::launch_lbvh_check((int)h->n, h->bvh, d, s));
+    HIPCHK(h, rg::launch_lbvh_check((int)h->n, h->bvh, h->leafbox, d, s));
     unsigned long long e = 0;
     HIPCHK(h, hipMemcpyAsync(&e, d, sizeof(e), hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));

@@ -212,7 +212,8 @@ __global__ void k_karras(int n, const unsigned long long* __restrict__ keys, int
 
 // DFS preorder renumbering + threaded links.
 __global__ void k_preorder(int n, const int* __restrict__ kl, const int* __restrict__ kf, const int* __restrict__ kla,
-                           const int* __restrict__ kpar, float4* nodes, int* parent, int* right, int* leaf_node) {
+                           const int* __restrict__ kpar, float4* nodes, int* parent, int* right, int* leaf_node,
+                           int* kpre) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int total = 2 * n - 1;
     if (x >= total) return;
@@ -255,36 +256,69 @@ __global__ void k_preorder(int n, const int* __restrict__ kl, const int* __restr
         right[ppre] = pre;
     }
     if (!internal) leaf_node[leafk] = pre;
+    else kpre[x] = pre;
 }
 
 // ---------------------------------------------------------------- refit ----------------------
 
+// Leaf k's node (preorder index p): its box, miss link p + 1, slot k.
+__device__ __forceinline__ void refit_leaf(int k, int p, const float4* __restrict__ leafbox, float4* nodes) {
+    const float4 lo = leafbox[2 * k], hi = leafbox[2 * k + 1];
+    nodes[2 * p] = make_float4(lo.x, lo.y, lo.z, __int_as_float(p + 1));
+    nodes[2 * p + 1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(k));
+}
+
+// Internal node q: the union of its children's boxes (left child q + 1, right child right[q]),
+// keeping its miss link.
+__device__ __forceinline__ void refit_internal(int q, float4* nodes, const int* __restrict__ right) {
+    const int a = q + 1, b = right[q];
+    const float4 alo = nodes[2 * a], ahi = nodes[2 * a + 1];
+    const float4 blo = nodes[2 * b], bhi = nodes[2 * b + 1];
+    const float qmiss = nodes[2 * q].w;
+    nodes[2 * q] = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z), qmiss);
+    nodes[2 * q + 1] = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z), __int_as_float(-1));
+}
+
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_int(a) == __float_as_int(b); }
+
+// Refit of any tree in one launch: each leaf climbs towards the root; the second of two siblings
+// to arrive at a parent (a flag per node) computes it and continues.
 __global__ void k_refit(int n, const float4* __restrict__ leafbox, float4* nodes, const int* __restrict__ parent,
                         const int* __restrict__ right, int* counter, const int* __restrict__ leaf_node) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int p = leaf_node[k];
-    const float4 lo = leafbox[2 * k], hi = leafbox[2 * k + 1];
-    nodes[2 * p] = make_float4(lo.x, lo.y, lo.z, __int_as_float(p + 1));
-    nodes[2 * p + 1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(k));
+    refit_leaf(k, p, leafbox, nodes);
     for (int guard = 0; guard < 192; ++guard) {  // tree depth <= 96 (augmented 96-bit keys)
         const int q = parent[p];
         if (q < 0) break;
         __threadfence();
         if (atomicAdd(&counter[q], 1) == 0) return;  // sibling not finished: it continues
         __threadfence();
-        const int a = q + 1, b = right[q];
-        const float4 alo = nodes[2 * a], ahi = nodes[2 * a + 1];
-        const float4 blo = nodes[2 * b], bhi = nodes[2 * b + 1];
-        const float qmiss = nodes[2 * q].w;
-        nodes[2 * q] = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z), qmiss);
-        nodes[2 * q + 1] = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z), __int_as_float(-1));
+        refit_internal(q, nodes, right);
         p = q;
     }
 }
 
+// Level-synchronous refit of a binned-SAH tree: the leaves, then one launch per level of internal
+// nodes (Karras ids first .. first + count - 1), deepest level first; no flags, no fences.
+__global__ void k_refit_leaves(int n, const float4* __restrict__ leafbox, float4* nodes,
+                               const int* __restrict__ leaf_node) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) refit_leaf(k, leaf_node[k], leafbox, nodes);
+}
+
+__global__ void k_refit_level(int first, int count, const int* __restrict__ kpre, float4* nodes,
+                              const int* __restrict__ right) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) refit_internal(kpre[first + i], nodes, right);
+}
+
+// Structure and boxes of the tree: links consistent, each leaf's box its Gaussian's, each internal
+// box exactly the union of its children's (what every refit computes).
 __global__ void k_check(int n, const float4* __restrict__ nodes, const int* __restrict__ right,
-                        const int* __restrict__ leaf_node, unsigned long long* errors) {
+                        const int* __restrict__ leaf_node, const float4* __restrict__ leafbox,
+                        unsigned long long* errors) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int total = 2 * n - 1;
     if (p >= total) return;
@@ -293,15 +327,23 @@ __global__ void k_check(int n, const float4* __restrict__ nodes, const int* __re
     unsigned long long e = 0;
     if (miss <= p || miss > total) e++;
     if (leaf >= 0) {
-        if (leaf >= n || leaf_node[leaf] != p || miss != p + 1) e++;
+        if (leaf >= n || leaf_node[leaf] != p || miss != p + 1) {
+            e++;
+        } else {
+            const float4 blo = leafbox[2 * leaf], bhi = leafbox[2 * leaf + 1];
+            const bool same = same_bits(lo.x, blo.x) && same_bits(lo.y, blo.y) && same_bits(lo.z, blo.z) &&
+                              same_bits(hi.x, bhi.x) && same_bits(hi.y, bhi.y) && same_bits(hi.z, bhi.z);
+            if (!same) e++;
+        }
     } else {
         const int a = p + 1, b = right[p];
         if (a >= total || b <= a || b >= total) {
             e++;
         } else {
             const float4 alo = nodes[2 * a], ahi = nodes[2 * a + 1], blo = nodes[2 * b], bhi = nodes[2 * b + 1];
-            const bool ok = lo.x <= fminf(alo.x, blo.x) && lo.y <= fminf(alo.y, blo.y) && lo.z <= fminf(alo.z, blo.z) &&
-                            hi.x >= fmaxf(ahi.x, bhi.x) && hi.y >= fmaxf(ahi.y, bhi.y) && hi.z >= fmaxf(ahi.z, bhi.z);
+            const bool ok = same_bits(lo.x, fminf(alo.x, blo.x)) && same_bits(lo.y, fminf(alo.y, blo.y)) &&
+                            same_bits(lo.z, fminf(alo.z, blo.z)) && same_bits(hi.x, fmaxf(ahi.x, bhi.x)) &&
+                            same_bits(hi.y, fmaxf(ahi.y, bhi.y)) && same_bits(hi.z, fmaxf(ahi.z, bhi.z));
             if (!ok) e++;
             if (__float_as_int(ahi.w) < 0 && __float_as_int(alo.w) != b) e++;  // left subtree ends at right
             if (__float_as_int(ahi.w) >= 0 && b != a + 1) e++;
@@ -350,6 +392,7 @@ int launch_radix_sort(int n, int passes, unsigned long long* keys[2], int* vals[
 }
 
 hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, BvhBuffers& b, hipStream_t s) {
+    b.levels = 0;  // Karras ids are not grouped by depth: the refit climbs with flags
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, s, n, leafbox, box, b.keys[0], b.vals[0]);
     const int cur = launch_radix_sort(n, 8, b.keys, b.vals, b.hist, s);
@@ -367,7 +410,7 @@ hipError_t launch_lbvh_finish(int n, const int* sorted_vals, BvhBuffers& b, hipS
     const int total = 2 * n - 1;
     hipLaunchKernelGGL(k_set_perm, dim3(blocks(n, 256)), dim3(256), 0, s, n, sorted_vals, b.perm, b.iperm);
     hipLaunchKernelGGL(k_preorder, dim3(blocks(total, 256)), dim3(256), 0, s, n, b.k_left, b.k_first, b.k_last,
-                       b.k_parent, b.nodes, b.parent, b.right, b.leaf_node);
+                       b.k_parent, b.nodes, b.parent, b.right, b.leaf_node, b.k_pre);
     return hipGetLastError();
 }
 
@@ -379,6 +422,17 @@ hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s) {
 
 hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (b.levels > 0) {
+        hipLaunchKernelGGL(k_refit_leaves, dim3(blocks(n, 256)), dim3(256), 0, s, n, leafbox_sorted, b.nodes,
+                           b.leaf_node);
+        for (int d = b.levels - 1; d >= 0; --d) {
+            const int first = b.level_start[d], count = b.level_start[d + 1] - first;
+            if (count > 0)
+                hipLaunchKernelGGL(k_refit_level, dim3(blocks(count, 256)), dim3(256), 0, s, first, count, b.k_pre,
+                                   b.nodes, b.right);
+        }
+        return hipGetLastError();
+    }
     const int total = 2 * n - 1;
     hipLaunchKernelGGL(k_fill, dim3(blocks(total, 256)), dim3(256), 0, s, b.counter, total, 0);
     hipLaunchKernelGGL(k_refit, dim3(blocks(n, 256)), dim3(256), 0, s, n, leafbox_sorted, b.nodes, b.parent, b.right,
@@ -393,10 +447,12 @@ hipError_t launch_tree_area(int n, const BvhBuffers& b, double* out, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s) {
+hipError_t launch_lbvh_check(int n, const BvhBuffers& b, const float4* leafbox_sorted, unsigned long long* errors,
+                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int total = 2 * n - 1;
-    hipLaunchKernelGGL(k_check, dim3(blocks(total, 256)), dim3(256), 0, s, n, b.nodes, b.right, b.leaf_node, errors);
+    hipLaunchKernelGGL(k_check, dim3(blocks(total, 256)), dim3(256), 0, s, n, b.nodes, b.right, b.leaf_node,
+                       leafbox_sorted, errors);
     return hipGetLastError();
 }
 

@@ -99,6 +99,8 @@ hipError_t launch_export_predicate(int n, const int* iperm, const float4* geo, f
                                    float* hlog, hipStream_t s);
 
 // ---- launchers (lbvh.hip) ---------------------------------------------------------------------
+// Depth bound of the binned-SAH build (its level loop fails beyond it).
+constexpr int kTreeMaxLevels = 201;
 struct BvhBuffers {
     unsigned long long* keys[2];
     int* vals[2];
@@ -118,6 +120,13 @@ struct BvhBuffers {
     float4* sah_prims[2];   // binned-SAH build: (box, original index) records, ping-pong
     float4* sah_tasks[2];   // binned-SAH build: node tasks of one level, ping-pong
     int* sah_ctr;           // binned-SAH build: counters
+    int* k_pre;             // Karras internal node id -> preorder index
+    // Binned-SAH trees number their internal nodes level by level: the nodes of depth d have the
+    // Karras ids [level_start[d], level_start[d + 1]), so a refit runs one launch per level,
+    // deepest first.  levels == 0 for an LBVH, whose refit climbs from the leaves with a flag per
+    // node.
+    int levels = 0;
+    int level_start[kTreeMaxLevels + 2] = {};
 };
 hipError_t launch_lbvh_build(int n, const float4* leafbox, const float* box, BvhBuffers& b, hipStream_t s);
 // Binned-SAH build (sah.hip): same outputs as launch_lbvh_build (geo: the preprocessed records in
@@ -128,7 +137,8 @@ hipError_t launch_sah_build(int n, const float4* leafbox, const float4* geo, con
 int64_t sah_task_bytes(int n);
 hipError_t launch_lbvh_finish(int n, const int* sorted_vals, BvhBuffers& b, hipStream_t s);
 hipError_t launch_lbvh_refit(int n, const float4* leafbox_sorted, BvhBuffers& b, hipStream_t s);
-hipError_t launch_lbvh_check(int n, const BvhBuffers& b, unsigned long long* errors, hipStream_t s);
+hipError_t launch_lbvh_check(int n, const BvhBuffers& b, const float4* leafbox_sorted, unsigned long long* errors,
+                             hipStream_t s);
 // Sum of the node boxes' surface areas, FP64, added (atomically) to *out.
 hipError_t launch_tree_area(int n, const BvhBuffers& b, double* out, hipStream_t s);
 hipError_t launch_iota(int n, int* perm, int* iperm, hipStream_t s);

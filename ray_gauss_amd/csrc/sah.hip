@@ -353,10 +353,13 @@ int64_t sah_task_bytes(int n) { return sah_task_capacity(n) * (int64_t)sizeof(Sa
 
 // Binned-SAH build into the Karras-form arrays of b (then the shared preorder renumbering), with
 // the slot order in b.perm / b.iperm.  Reads one counter back per tree level (synchronises the
-// stream): not for graph capture.  Returns hipErrorUnknown when a consistency check failed.
+// stream): not for graph capture.  Internal node ids are handed out level by level, so the
+// counter read after each level also gives the id range of the next level (b.levels,
+// b.level_start: the refit's launches).  Returns hipErrorUnknown when a consistency check failed.
 hipError_t launch_sah_build(int n, const float4* leafbox, const float4* geo, const float* box, BvhBuffers& b,
                             hipStream_t s) {
     if (n <= 1) return launch_lbvh_build(n, leafbox, box, b, s);
+    b.levels = 0;
     const int total = 2 * n - 1;
     const int max_tasks = (int)sah_task_capacity(n);
     hipLaunchKernelGGL(k_sah_fill, dim3(blocks(total, 256)), dim3(256), 0, s, b.k_parent, total, -1);
@@ -370,9 +373,11 @@ hipError_t launch_sah_build(int n, const float4* leafbox, const float4* geo, con
     hipError_t e;
     if ((e = hipMemcpyAsync(host, ctr, sizeof host, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-    int pp = 0;
+    int pp = 0, levels = 0;
+    b.level_start[0] = 0;
+    b.level_start[1] = host[1];  // the root
     for (int level = 0; host[2] > 0; ++level) {
-        if (level > 200 || host[3] != 0) return hipErrorUnknown;
+        if (level >= kTreeMaxLevels || host[3] != 0) return hipErrorUnknown;
         const int ntasks = host[2];
         hipLaunchKernelGGL(k_sah_fill, dim3(1), dim3(64), 0, s, ctr + 2, 1, 0);
         hipLaunchKernelGGL(k_sah_level, dim3(ntasks), dim3(kSahBlock), 0, s, n, tasks[pp], prims[pp], prims[pp ^ 1],
@@ -383,10 +388,14 @@ hipError_t launch_sah_build(int n, const float4* leafbox, const float4* geo, con
         // only; leaves are written to vals_out when they are made, so nothing else is needed
         if ((e = hipMemcpyAsync(host, ctr, sizeof host, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        b.level_start[level + 2] = host[1];  // ids made by this level: the next level's nodes
+        levels = level + 1;
         pp ^= 1;
     }
     if (host[3] != 0 || host[1] != n - 1) return hipErrorUnknown;
-    return launch_lbvh_finish(n, vals_out, b, s);
+    if ((e = launch_lbvh_finish(n, vals_out, b, s)) != hipSuccess) return e;
+    b.levels = levels;
+    return hipSuccess;
 }
 
 }  // namespace rg

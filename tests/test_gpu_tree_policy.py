@@ -33,7 +33,8 @@ def test_tree_cost_of_identical_gaussians_is_the_node_count(gpu_device, builder)
     assert rc.tree_cost() == pytest.approx(2 * n - 1, rel=1e-12)
 
 
-def test_refit_quality_rebuild_policy(gpu_device):
+@pytest.mark.parametrize("builder", [0, 1])
+def test_refit_quality_rebuild_policy(gpu_device, builder):
     sc = scenes.scene_blobs(20000, 16, seed=5, num_sg=2)
     prm = scenes.RenderParams(dt=0.01, t_eps=1e-4, white_background=True)
     cams = [scenes.look_at_camera(scenes.orbit_eye(4.0, a, 20.0), width=64, height=48).launch_record()
@@ -42,7 +43,8 @@ def test_refit_quality_rebuild_policy(gpu_device):
     gen = torch.Generator(device=gpu_device).manual_seed(3)
     target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
     t = to_dev(sc, gpu_device)
-    tr = BatchedTrainer(RayCaster(prm, TuneConfig()), t, cams, rv, rp, target, use_graph=True, rebuild_ratio=1.2,
+    tune = TuneConfig(tree_builder=builder)
+    tr = BatchedTrainer(RayCaster(prm, tune), t, cams, rv, rp, target, use_graph=True, rebuild_ratio=1.2,
                         check_every=1)
     base = tr.base_cost
     assert base > 1.0
@@ -54,16 +56,21 @@ def test_refit_quality_rebuild_policy(gpu_device):
     first = tr.graph
     tr.step()
     assert tr.rebuilds == 1
-    fresh = RayCaster(prm, TuneConfig())
+    assert tr.rc.check_bvh() == 0  # the refitted tree: every box exactly its children's union
+    fresh = RayCaster(prm, tune)
     fresh.prepare(t["positions"], t["scales"], t["quats"], t["densities"], t["sh"], t["sg"], t["sharpness"],
                   t["lobe_axis"], rebuild=True)
     assert tr.base_cost == pytest.approx(fresh.tree_cost(), rel=1e-9)
-    assert tr.graph is first  # same tensors and buffers: the captured graph stays valid
-    eager = BatchedTrainer(RayCaster(prm, TuneConfig()), t, cams, rv, rp, target, use_graph=False,
+    if builder == 0:
+        assert tr.graph is first  # same tensors and buffers: the captured graph stays valid
+    else:
+        assert tr.graph is None  # the SAH tree's refit launches follow its levels: capture again
+    eager = BatchedTrainer(RayCaster(prm, tune), t, cams, rv, rp, target, use_graph=False,
                            rebuild_ratio=None)
     ge = {k: (v.clone() if v is not None else None) for k, v in eager.step().items()}
     g = tr.step()
     torch.cuda.synchronize()
     assert tr.rebuilds == 1
+    assert tr.graph is not None
     for k in ("positions", "scales", "quats", "densities", "sh", "sg"):
         assert max_rel(g[k].cpu().numpy(), ge[k].cpu().numpy()) <= 1e-5, k
