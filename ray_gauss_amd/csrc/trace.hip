@@ -197,38 +197,38 @@ __device__ __forceinline__ void composite(const float* sig, const float* cr, con
     }
 }
 
-// Per-sample backward coefficients of one slab, from its buffers: overwrites (sig, cr, cg) with
-// (A_i, Bq_i, E_i) and advances T and the colour remainder Crem past the slab.
-//   A_i  = dt * dLdC . (T_i c^_i - Crem_i)      (dcolor_dsigma_aux, backward :706)
+// Per-sample backward coefficients of one slab, from its buffers: overwrites (sig, cr) with
+// (G_i, Bq_i) and advances T and the projected colour remainder Dr = dLdC . Crem past the
+// slab (Crem_i: the ray's colour not yet composited before sample i; only its projection on
+// dLdC enters the coefficients, so one scalar is carried instead of three).
+//   A_i  = dt * dLdC . (T_i c^_i - Crem_i) = dt * (T_i E_i - Dr_i)   (dcolor_dsigma_aux, backward :706)
 //   Bq_i = T_i alpha_i / sigma_i                (0 when sigma_i == 0)
-//   E_i  = dLdC . c^_i
+//   E_i  = dLdC . c^_i = (dLdC . sums_i) / sigma_i
+//   G_i  = A_i - Bq_i E_i, so that a hit of colour c weighs sample i by phi_i = G_i + Bq_i (dLdC . c)
 template <int SPB>
 __device__ __forceinline__ void slab_coeffs(float* sig, float* cr, float* cg, float* cb, const DevParams& p,
-                                            const float* dL, float& T, float* Crem) {
+                                            const float* dL, float& T, float& Dr) {
 #pragma unroll
     for (int i = 0; i < SPB; ++i) {
         if (SPB > 8 && i >= p.spb) break;
         const float s = sig[i];
         const float alpha = sample_alpha(s, p.dt);
-        // the forward's colour = sums / sigma, evaluated as there (composite)
+        // the forward's colour is sums / sigma (composite); only its projection E on dLdC enters
         const float inv = s > 0.0f ? __builtin_amdgcn_rcpf(s) : 0.0f;
-        const float r = cr[i] * inv, g = cg[i] * inv, b = cb[i] * inv;
-        const float A = p.dt * (dL[0] * (T * r - Crem[0]) + dL[1] * (T * g - Crem[1]) + dL[2] * (T * b - Crem[2]));
-        const float Bq = T * alpha * inv;
-        const float E = dL[0] * r + dL[1] * g + dL[2] * b;
+        const float E = (dL[0] * cr[i] + dL[1] * cg[i] + dL[2] * cb[i]) * inv;
+        const float A = p.dt * (T * E - Dr);
         const float ta = T * alpha;
-        Crem[0] -= ta * r;
-        Crem[1] -= ta * g;
-        Crem[2] -= ta * b;
+        const float Bq = ta * inv;
+        Dr -= ta * E;
         T *= 1.0f - alpha;
-        sig[i] = A;
+        sig[i] = A - Bq * E;
         cr[i] = Bq;
-        cg[i] = E;
     }
 }
 
 // Gradient moments of one (slab, hit) for one ray.  Over the slab's samples the hit contributes
-// four sums (Ss = sum w phi, S1 = sum w phi tau, S2 = sum w phi tau^2, Sc = sum w Bq; tau = t - t_c).
+// four sums (Ss = sum w phi, S1 = sum w phi tau, S2 = sum w phi tau^2, Sc = sum w Bq; tau = t - t_c,
+// phi = G + Bq F with F = dLdC . colour; G, Bq from slab_coeffs).
 // Every gradient of the Gaussian is linear in the ray-dependent moments
 //   Gy'   = P S0 + D T1                                     (3)
 //   Y'    = P P^T S0 + (P D^T + D P^T) T1 + D D^T T2         (6, symmetric)
@@ -240,8 +240,8 @@ constexpr int kMoments = 13;
 
 template <int SPB, bool RNG>
 __device__ __forceinline__ void grad_moments(const Isect& I, float sigma, float4 col, float tb, unsigned int seed,
-                                             const DevParams& p, const float* Aq, const float* Bq, const float* Eq,
-                                             const float* dL, float* v) {
+                                             const DevParams& p, const float* Gq, const float* Bq, const float* dL,
+                                             float* v) {
     const float F = dL[0] * col.x + dL[1] * col.y + dL[2] * col.z;
     float Ss = 0.0f, S1 = 0.0f, S2 = 0.0f, Sc = 0.0f;
     unsigned int st = seed;
@@ -253,7 +253,7 @@ __device__ __forceinline__ void grad_moments(const Isect& I, float sigma, float4
         const float tau = t - I.tc;
         const float w0 = weight(k, tau);
         const float w = sigma * w0 > p.sigma_thr ? w0 : 0.0f;  // below the threshold: exact zeros
-        const float phi = Aq[i] + Bq[i] * (F - Eq[i]);
+        const float phi = Gq[i] + Bq[i] * F;
         const float wp = w * phi;
         Ss += wp;
         S1 += wp * tau;
@@ -858,7 +858,7 @@ template <int SPB, bool RNG>
 __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool dsc, const Ray& ray,
                                     const float4* __restrict__ rgbv, float cur_tb, unsigned int cur_seed,
                                     float cur_tmin, float cur_tmax,
-                                    const float* Aq, const float* Bq, const float* Eq, const float* dL, GradStage& G,
+                                    const float* Gq, const float* Bq, const float* dL, GradStage& G,
                                     int& nst, int lane, const DevGrad& gr, int view, unsigned long long* drops) {
     const float ix = 1.0f / ray.dx, iy = 1.0f / ray.dy, iz = 1.0f / ray.dz;
     const int total = sc.total_nodes;
@@ -891,7 +891,7 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
             Isect I;
             if (intersect(g0, g1, g2, g3, ray, I) && overlaps(cur_tmin, cur_tmax, I.r1, I.r2)) {
                 refine_sampling(g0, g1, g2, g3, ray, I);
-                grad_moments<SPB, RNG>(I, g0.w, rgbv[leaf], cur_tb, cur_seed, p, Aq, Bq, Eq, dL, v);
+                grad_moments<SPB, RNG>(I, g0.w, rgbv[leaf], cur_tb, cur_seed, p, Gq, Bq, dL, v);
                 c = true;
             }
         }
@@ -911,9 +911,9 @@ __device__ void direct_scatter_wave(const DevScene& sc, const DevParams& p, bool
 
 // Backward of one ray per lane, called by all 64 lanes of the wave together (inactive lanes pass
 // active = false).  Each outer iteration every live lane prepares its next slab that has hits
-// (recomputed buffers, per-sample coefficients, T and Crem advanced); then the wave walks the
-// union of the prepared slabs' hits in increasing (view, slot) order and scatters one summed
-// contribution per Gaussian.  Slabs served by per-slab traversal scatter per lane.
+// (recomputed buffers, per-sample coefficients, T and the colour remainder advanced); then the
+// wave walks the union of the prepared slabs' hits in increasing (view, slot) order and scatters
+// one summed contribution per Gaussian.  Slabs served by per-slab traversal scatter per lane.
 template <int SPB, int CAP, bool RNG, int RM>
 __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool active, const Ray& ray, unsigned int seed,
                               const float4* __restrict__ rgbv, const float* Cfwd, const float* dL, IList<CAP>& L,
@@ -922,7 +922,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
     int nst = 0;  // records in the staging area
     bool alive = active && (ray.tenter < ray.texit) && (add_exact(ray.tenter, p.dt * (float)p.spb) > ray.tenter);
     float T = 1.0f;
-    float Crem[3] = {Cfwd[0], Cfwd[1], Cfwd[2]};
+    float Dr = dL[0] * Cfwd[0] + dL[1] * Cfwd[1] + dL[2] * Cfwd[2];  // dLdC . (colour not yet composited)
     SegState s;
     seg_init(s, ray, p);
     const int key_base = view * sc.n;
@@ -940,8 +940,8 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         bool cur_dlist = false;  // the entries are a direct slab's recorded hits (all in the slab)
         int cur_dpos = 0;
         unsigned int cur_m = 0u;  // list mode: the entries in the slab (slab_mask), not yet scattered
-        // the prepared slab's sample buffers; after slab_coeffs sig, cr, cg hold the per-sample
-        // backward coefficients A_i, Bq_i, E_i read by the scatter below
+        // the prepared slab's sample buffers; after slab_coeffs sig and cr hold the per-sample
+        // backward coefficients G_i, Bq_i read by the scatter below
         float sig[SPB], cr[SPB], cg[SPB], cb[SPB];
         if (alive && !pend) {
             if (!(T > p.t_eps)) {
@@ -1006,7 +1006,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     }
                 }
                 if (nh > 0) {
-                    slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Crem);
+                    slab_coeffs<SPB>(sig, cr, cg, cb, p, dL, T, Dr);
                     cur_tb = tb;
                     cur_seed = seed;
                     cur_tmin = tmin;
@@ -1029,7 +1029,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         }
         // converged: slabs whose hit set overflowed the list scatter from a packet traversal
         if (__ballot(dsc) != 0ULL)
-            direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, cg, dL, G,
+            direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, dL, G,
                                      nst, lane, gr, view, drops);
         // wave-merged scatter over the prepared slabs' list hits.  A lane's key (view * n + slot of
         // its next hit; INT_MAX when it has none) changes only when its hit is scattered, and the
@@ -1072,7 +1072,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                     float b, c;
                     isect_terms(g0, g1, g2, g3, ray, I, b, c);
                     refine_sampling(g0, g1, g2, g3, ray, I);
-                    grad_moments<SPB, RNG>(I, g0.w, rgbv[slot], cur_tb, cur_seed, p, sig, cr, cg, dL, v);
+                    grad_moments<SPB, RNG>(I, g0.w, rgbv[slot], cur_tb, cur_seed, p, sig, cr, dL, v);
                 }
                 if (cur_dlist) {
                     ++e;
