@@ -263,14 +263,19 @@ __device__ __forceinline__ void grad_moments(const Isect& I, float sigma, float4
     const float S0 = sigma * Ss, T1 = sigma * S1, T2 = sigma * S2;
     const float P[3] = {I.Px, I.Py, I.Pz};
     const float D[3] = {I.Dx, I.Dy, I.Dz};
+    // Gy' = U = P S0 + D T1; with W = P T1 + D T2, Y'_ab = P_a U_b + D_a W_b (symmetric)
+    float W[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v[a] = P[a] * S0 + D[a] * T1;
+    for (int a = 0; a < 3; ++a) {
+        v[a] = P[a] * S0 + D[a] * T1;
+        W[a] = P[a] * T1 + D[a] * T2;
+    }
     // Y' entries (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
     const int ya[6] = {0, 1, 2, 0, 0, 1}, yb[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
     for (int e = 0; e < 6; ++e) {
         const int a = ya[e], b = yb[e];
-        v[3 + e] = P[a] * P[b] * S0 + (P[a] * D[b] + D[a] * P[b]) * T1 + D[a] * D[b] * T2;
+        v[3 + e] = P[a] * v[b] + D[a] * W[b];
     }
     v[9] = Ss;
     const float sc_ = sigma * Sc;
