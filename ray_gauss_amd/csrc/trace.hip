@@ -491,6 +491,7 @@ struct SegState {
     int dpos;       // replay of a direct slab: first word of its recorded hit list
     int dcnt;       //   and its length (-1: no recorded list, traverse)
     bool dense;     // the last direct slab held more hits than the list: walk the next one too
+    int knext;      // replay: slab index of the next record not yet taken (INT_MAX: none)
 };
 
 __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevParams& p) {
@@ -505,6 +506,7 @@ __device__ __forceinline__ void seg_init(SegState& s, const Ray& ray, const DevP
     s.dpos = 0;
     s.dcnt = -1;
     s.dense = false;
+    s.knext = INT_MAX;
 }
 
 // ------------------------------------------------------------------ hit-list records ---------
@@ -600,6 +602,7 @@ __device__ __forceinline__ void rec_direct_close(Rec& rc, int start, int n, int 
 template <int CAP>
 __device__ __forceinline__ void take_records(SegState& s, const DevScene& sc, const Ray& ray, IList<CAP>& L,
                                              int lane, Rec& rc) {
+    s.knext = INT_MAX;
     while (rc.pos + 8 <= rc.end) {
         const u32x4 h = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(rec_at(rc, rc.pos)));
         if (!(h.x & kRecOn)) {
@@ -607,7 +610,10 @@ __device__ __forceinline__ void take_records(SegState& s, const DevScene& sc, co
             s.valid = false;
             return;
         }
-        if ((int)(h.x & (unsigned int)kRecMaxSlab) > s.k) return;
+        if ((int)(h.x & (unsigned int)kRecMaxSlab) > s.k) {
+            s.knext = (int)(h.x & (unsigned int)kRecMaxSlab);
+            return;
+        }
         const int cnt = (int)((h.x >> 24) & 63u);
         const int len = rec_len(cnt);
         if (h.x & kRecDirect) {
@@ -935,10 +941,37 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
     // 4 % faster than one, the same as a quarter; one slab 12 % faster on a 300k-Gaussian batch)
     const float window = p.bwd_window * p.dt * (float)p.spb;
     bool pend = false;  // next_slab() has produced a slab not yet processed
+    bool jump = false;  // the last round found no hit in any lane's slab
+    const float slab = p.dt * (float)p.spb;
     float tmin = 0.0f, tmax = 0.0f;
     while (__ballot(alive) != 0ULL) {
+        // After a round without hits the lanes that step through a list step at once over the
+        // slabs empty for them, up to the wavefront's earliest possible next hit: the smallest
+        // start (or the segment end) among the entries that have not ended before each lane's
+        // slab.  Every slab passed ends before that point, so none overlaps an entry; the slabs
+        // with hits and their order are unchanged, and the lanes stay in step in depth.  A
+        // replaying lane stops at the slab of its next record, which it takes there.
+        if (jump) {
+            const bool step = alive && !pend && s.valid && !s.direct;
+            float bound = alive ? s.tb : INFINITY;
+            if (step) {
+                bound = s.seg_hi;
+                for (int e = 0; e < s.cnt; ++e)
+                    if (!(L.r2[e][lane] < s.tb)) bound = fminf(bound, fmaxf(L.r1[e][lane], s.tb));
+            }
+            const float tstar = wave_min64f(bound);
+            if (step && tstar < INFINITY) {
+                const int kn = rc.w ? s.knext : INT_MAX;
+                while (s.k < kn) {
+                    const float nt = add_exact(s.tb, slab);
+                    if (!(fminf(ray.texit, nt) < tstar) || !(nt > s.tb)) break;
+                    s.tb = nt;
+                    ++s.k;
+                }
+            }
+        }
         // prepared slab: list mode (has: cur_cnt list entries) or per-slab traversal (dsc)
-        bool has = false, dsc = false;
+        bool has = false, dsc = false, prepped = false;
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;
         unsigned int cur_seed = 0u;
         int cur_cnt = 0;
@@ -962,6 +995,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
         const float tsync = wave_min64f(alive && pend ? s.tb : INFINITY);
         if (alive && pend && s.tb <= tsync + window) {
             pend = false;
+            prepped = true;
             {
                 const float tb = s.tb;
 #pragma unroll
@@ -1032,6 +1066,7 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
                 if (!advance_slab(s, p)) alive = false;
             }
         }
+        jump = __ballot(prepped) != 0ULL && __ballot(has || dsc) == 0ULL;
         // converged: slabs whose hit set overflowed the list scatter from a packet traversal
         if (__ballot(dsc) != 0ULL)
             direct_scatter_wave<SPB, RNG>(sc, p, dsc, ray, rgbv, cur_tb, cur_seed, cur_tmin, cur_tmax, sig, cr, dL, G,
