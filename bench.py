@@ -162,7 +162,8 @@ def full_step(args):
     tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
                       replay_words=args.replay_words, tree_builder=args.tree_builder,
                       ray_chunk=args.ray_chunk, refill_lanes=args.refill_lanes,
-                      bwd_ray_chunk=args.bwd_ray_chunk, bwd_window=args.bwd_window)
+                      bwd_ray_chunk=args.bwd_ray_chunk, bwd_window=args.bwd_window,
+                      fwd_jump=not args.fwd_round_per_slab)
     trainer = BatchedTrainer(RayCaster(wl.params, tune), buf, cams, rv, rp, target, loss_kind=0,
                              use_graph=not args.no_graph)
     opts = [FusedAdam([raw[k]], lr=FULL_STEP_LR[k]) for k in raw]
@@ -233,6 +234,8 @@ def main():
     ap.add_argument("--bwd-ray-chunk", type=int, default=0, help="backward launch, as --ray-chunk")
     ap.add_argument("--bwd-window", type=float, default=0.5,
                     help="backward depth synchronisation in slabs (lanes within it of the laggard step together)")
+    ap.add_argument("--fwd-round-per-slab", action="store_true",
+                    help="forward: one round per slab (no joint step over slabs empty for every lane)")
     ap.add_argument("--refill-lanes", type=int, default=0,
                     help="persistent forward: hand new rays to idle lanes once this many are idle (0: whole chunks)")
     ap.add_argument("--autotune", action="store_true",
@@ -295,7 +298,8 @@ def main():
     tune = TuneConfig(args.hit_depth, args.seg_slabs, packet=not args.no_packet, fwd_hit_depth=args.fwd_hit_depth,
                       replay_words=args.replay_words, tree_builder=args.tree_builder,
                       ray_chunk=args.ray_chunk, refill_lanes=args.refill_lanes,
-                      bwd_ray_chunk=args.bwd_ray_chunk, bwd_window=args.bwd_window)
+                      bwd_ray_chunk=args.bwd_ray_chunk, bwd_window=args.bwd_window,
+                      fwd_jump=not args.fwd_round_per_slab)
     rc = None  # the per-view caster, made after the batched run so that its memory is not counted there
 
     def prepare(rebuild=False):
@@ -421,7 +425,8 @@ def main():
                    "packet_traversal": tune.packet, "fwd_hit_depth": tune.fwd_hit_depth,
                    "replay_words": tune.replay_words, "tree_builder": ("lbvh", "sah")[tune.tree_builder],
                    "ray_chunk": tune.ray_chunk, "refill_lanes": tune.refill_lanes,
-                   "bwd_ray_chunk": tune.bwd_ray_chunk, "bwd_window": tune.bwd_window},
+                   "bwd_ray_chunk": tune.bwd_ray_chunk, "bwd_window": tune.bwd_window,
+                   "fwd_jump": tune.fwd_jump},
         "tree_build_ms": build_ms,
         "per_view_rays_per_s": (R / dt_pv) if dt_pv else None,
         "batched_over_per_view": (dt_pv / dt_b) if dt_pv else None,

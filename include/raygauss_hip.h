@@ -77,6 +77,10 @@ typedef struct rg_config {
     float bwd_window;          /* backward: lanes within this many slabs of the wave's laggard
                                   prepare their next slab together and merge their gradients
                                   (<= 0: the default, 0.5; the best value depends on the scene) */
+    int32_t fwd_round_per_slab; /* forward: 0 (default) = after a round in which no lane of a
+                                  wavefront had a hit, the lanes stepping through a list step
+                                  together over the slabs empty for them (the backward always
+                                  does); 1 = one round per slab                                */
 } rg_config;
 
 /* One pinhole view: the launch record of optix_utils.py:166-181. u, v, w are columns 0..2 of

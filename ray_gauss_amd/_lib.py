@@ -23,7 +23,7 @@ class Config(C.Structure):
                 ("jitter", C.c_int32), ("rnd_sample", C.c_int32), ("iteration", C.c_uint32),
                 ("fwd_hit_depth", C.c_int32), ("replay_words", C.c_int32),
                 ("tree_builder", C.c_int32), ("ray_chunk", C.c_int32), ("refill_lanes", C.c_int32),
-                ("bwd_ray_chunk", C.c_int32), ("bwd_window", C.c_float)]
+                ("bwd_ray_chunk", C.c_int32), ("bwd_window", C.c_float), ("fwd_round_per_slab", C.c_int32)]
 
 
 class CameraS(C.Structure):

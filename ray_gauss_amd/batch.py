@@ -170,6 +170,7 @@ TUNE_AXES = {
     "ray_chunk": (0, 64, 128),
     "bwd_ray_chunk": (0, 64),
     "bwd_window": (0.5, 1.0, 2.0),
+    "fwd_jump": (True, False),
 }
 
 

@@ -35,7 +35,9 @@ class TuneConfig:
     bwd_ray_chunk: the same for the backward (persistent measured 9 % slower at 1M Gaussians);
     bwd_window: the backward's depth synchronisation, in slabs: lanes within this distance of the
     wave's laggard prepare their next slab together and merge their gradients (0.5 is best at 1M
-    Gaussians, 1.0 on the 300k-Gaussian batch of uneven views)."""
+    Gaussians, 1.0 on the 300k-Gaussian batch of uneven views); fwd_jump: after a round in which no
+    lane of a wavefront had a hit, the forward's lanes step together over the slabs empty for
+    them instead of one slab per round (2 % faster at 1M Gaussians; the backward always does)."""
 
     hit_depth: int = 16
     seg_slabs: int = 32
@@ -47,6 +49,7 @@ class TuneConfig:
     refill_lanes: int = 0
     bwd_ray_chunk: int = 0
     bwd_window: float = 0.5
+    fwd_jump: bool = True
 
 
 def _ptr(t):
@@ -132,6 +135,7 @@ class RayCaster:
         c.refill_lanes = int(getattr(self.tune, "refill_lanes", 0))
         c.bwd_ray_chunk = int(getattr(self.tune, "bwd_ray_chunk", 0))
         c.bwd_window = float(getattr(self.tune, "bwd_window", 0.5))
+        c.fwd_round_per_slab = 0 if getattr(self.tune, "fwd_jump", True) else 1
         c.jitter = int(getattr(p, "jitter", 0))
         c.rnd_sample = int(getattr(p, "rnd_sample", 0))
         c.iteration = int(getattr(p, "iteration", 0) or 0) & 0xFFFFFFFF

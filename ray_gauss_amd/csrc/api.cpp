@@ -143,6 +143,8 @@ int check_config(rg_handle* h, const rg_config* c) {
     if (c->bwd_ray_chunk != 0 && (c->bwd_ray_chunk < 64 || c->bwd_ray_chunk > 65536 || (c->bwd_ray_chunk & 63)))
         return fail(h, RG_ERR_ARG, "bwd_ray_chunk must be 0 or a multiple of 64 in 64..65536");
     if (!(c->bwd_window <= 1024.0f)) return fail(h, RG_ERR_ARG, "bwd_window must be at most 1024 slabs");
+    if (c->fwd_round_per_slab != 0 && c->fwd_round_per_slab != 1)
+        return fail(h, RG_ERR_ARG, "fwd_round_per_slab must be 0 or 1");
     return RG_OK;
 }
 
@@ -239,6 +241,7 @@ rg::DevParams dev_params(const rg_handle* h) {
     p.refill_lanes = h->cfg.refill_lanes;
     p.queue = h->queue;
     p.bwd_window = h->cfg.bwd_window > 0.0f ? h->cfg.bwd_window : 0.5f;
+    p.fwd_jump = h->cfg.fwd_round_per_slab ? 0 : 1;
     return p;
 }
 

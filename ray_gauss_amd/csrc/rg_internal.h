@@ -68,6 +68,8 @@ struct DevParams {
     unsigned long long* queue;
     // backward depth synchronisation, in slabs (lanes within it of the wave's laggard step together)
     float bwd_window;
+    // forward: step over slabs empty for every lane of a wavefront together (jump_empty)
+    int fwd_jump;
 };
 
 // Gradient accumulators (device-scope atomic adds; the finish kernel reads and clears them).

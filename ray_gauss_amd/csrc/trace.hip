@@ -784,6 +784,34 @@ __device__ __forceinline__ void flush_counters(const Counters& c, DevStats st) {
 
 // Starts a ray: clears its output and sets up its slab stepper; false when the ray has no slab
 // to process (it misses the scene box, or its first slab is below half an ulp of t).
+// Wavefront-wide step over slabs without hits, after a round in which no lane had one: the lanes
+// stepping through a list (step) advance together over the slabs empty for them, up to the
+// wavefront's earliest possible next hit -- the smallest start (or the segment end) among the list
+// entries that have not ended before each lane's slab; lanes that cannot step bound it by their
+// slab (bound_tb).  Every slab passed ends before that point, so none overlaps an entry: the slabs
+// with hits and their order are unchanged, and the lanes stay in step in depth.  A replaying lane
+// stops at the slab of its next record (knext), which it takes there.  Called by all 64 lanes.
+template <int CAP>
+__device__ __forceinline__ void jump_empty(SegState& s, const Ray& ray, const DevParams& p, const IList<CAP>& L,
+                                           int lane, bool step, bool bound_tb, int knext) {
+    float bound = bound_tb ? s.tb : INFINITY;
+    if (step) {
+        bound = s.seg_hi;
+        for (int e = 0; e < s.cnt; ++e)
+            if (!(L.r2[e][lane] < s.tb)) bound = fminf(bound, fmaxf(L.r1[e][lane], s.tb));
+    }
+    const float tstar = wave_min64f(bound);
+    if (step && tstar < INFINITY) {
+        const float slab = p.dt * (float)p.spb;
+        while (s.k < knext) {
+            const float nt = add_exact(s.tb, slab);
+            if (!(fminf(ray.texit, nt) < tstar) || !(nt > s.tb)) break;
+            s.tb = nt;
+            ++s.k;
+        }
+    }
+}
+
 __device__ __forceinline__ bool forward_begin(const DevParams& p, const Ray& ray, RayOut& o, SegState& s) {
     o.C[0] = o.C[1] = o.C[2] = 0.0f;
     o.depth = 0.0f;
@@ -799,7 +827,8 @@ __device__ __forceinline__ bool forward_begin(const DevParams& p, const Ray& ray
 template <int SPB, int CAP, bool RNG, int RM>
 __device__ __forceinline__ bool forward_step(const DevScene& sc, const DevParams& p, const Ray& ray,
                                              unsigned int& seed, const float4* __restrict__ rgbv, IList<CAP>& L,
-                                             int lane, Counters& ctr, RayOut& o, Rec& rc, SegState& s) {
+                                             int lane, Counters& ctr, RayOut& o, Rec& rc, SegState& s,
+                                             bool* hit = nullptr) {
     if (!(o.T > p.t_eps)) return false;
     const float first_tb = ray.tenter;
     sync_gathers(s, ray, p);
@@ -843,6 +872,7 @@ __device__ __forceinline__ bool forward_step(const DevScene& sc, const DevParams
         }
     }
     ctr.K += nh;
+    if (hit) *hit = nh > 0;
     if (p.max_hits > 0 && nh > p.max_hits) ctr.ovf++;
     if (nh > 0) {
         composite<SPB, RNG>(sig, cr, cg, cb, tb, seed, p, o);
@@ -856,8 +886,13 @@ __device__ void forward_ray(const DevScene& sc, const DevParams& p, const Ray& r
                             const float4* __restrict__ rgbv, IList<CAP>& L, int lane, Counters& ctr, RayOut& o,
                             Rec& rc) {
     SegState s;
-    if (!forward_begin(p, ray, o, s)) return;
-    while (forward_step<SPB, CAP, RNG, RM>(sc, p, ray, seed, rgbv, L, lane, ctr, o, rc, s)) {
+    bool alive = forward_begin(p, ray, o, s);
+    bool jump = false;
+    while (__ballot(alive) != 0ULL) {
+        if (jump) jump_empty<CAP>(s, ray, p, L, lane, alive && s.valid && !s.direct, alive, INT_MAX);
+        bool hit = false;
+        if (alive) alive = forward_step<SPB, CAP, RNG, RM>(sc, p, ray, seed, rgbv, L, lane, ctr, o, rc, s, &hit);
+        jump = p.fwd_jump && __ballot(hit) == 0ULL;
     }
 }
 
@@ -941,35 +976,11 @@ __device__ void backward_wave(const DevScene& sc, const DevParams& p, bool activ
     // 4 % faster than one, the same as a quarter; one slab 12 % faster on a 300k-Gaussian batch)
     const float window = p.bwd_window * p.dt * (float)p.spb;
     bool pend = false;  // next_slab() has produced a slab not yet processed
-    bool jump = false;  // the last round found no hit in any lane's slab
-    const float slab = p.dt * (float)p.spb;
+    bool jump = false;  // the last round found no hit in any lane's slab (jump_empty)
     float tmin = 0.0f, tmax = 0.0f;
     while (__ballot(alive) != 0ULL) {
-        // After a round without hits the lanes that step through a list step at once over the
-        // slabs empty for them, up to the wavefront's earliest possible next hit: the smallest
-        // start (or the segment end) among the entries that have not ended before each lane's
-        // slab.  Every slab passed ends before that point, so none overlaps an entry; the slabs
-        // with hits and their order are unchanged, and the lanes stay in step in depth.  A
-        // replaying lane stops at the slab of its next record, which it takes there.
-        if (jump) {
-            const bool step = alive && !pend && s.valid && !s.direct;
-            float bound = alive ? s.tb : INFINITY;
-            if (step) {
-                bound = s.seg_hi;
-                for (int e = 0; e < s.cnt; ++e)
-                    if (!(L.r2[e][lane] < s.tb)) bound = fminf(bound, fmaxf(L.r1[e][lane], s.tb));
-            }
-            const float tstar = wave_min64f(bound);
-            if (step && tstar < INFINITY) {
-                const int kn = rc.w ? s.knext : INT_MAX;
-                while (s.k < kn) {
-                    const float nt = add_exact(s.tb, slab);
-                    if (!(fminf(ray.texit, nt) < tstar) || !(nt > s.tb)) break;
-                    s.tb = nt;
-                    ++s.k;
-                }
-            }
-        }
+        if (jump)
+            jump_empty<CAP>(s, ray, p, L, lane, alive && !pend && s.valid && !s.direct, alive, rc.w ? s.knext : INT_MAX);
         // prepared slab: list mode (has: cur_cnt list entries) or per-slab traversal (dsc)
         bool has = false, dsc = false, prepped = false;
         float cur_tb = 0.0f, cur_tmin = 0.0f, cur_tmax = 0.0f;

@@ -191,6 +191,32 @@ def test_graph_replay_matches_eager(blobs, gpu_device):
             assert max_rel(g[k].cpu().numpy(), ge[k].cpu().numpy()) <= 1e-5, (rep, k)
 
 
+@pytest.mark.parametrize("words", [0, 1536])
+def test_joint_step_over_empty_slabs_changes_nothing(blobs, gpu_device, words):
+    """After a round in which no lane had a hit, the lanes step together over the slabs empty for
+    them (always in the backward; in the forward unless fwd_jump is off): the slabs with hits and
+    their order per ray are unchanged, so the image and loss are identical and the gradients equal
+    up to the order of atomic sums."""
+    sc, prm, cams, pixels, rc0, t0, rv, rp, offs = blobs
+    gen = torch.Generator(device=gpu_device).manual_seed(4)
+    target = torch.rand((rv.shape[0], 3), generator=gen, device=gpu_device)
+    res = {}
+    for jump in (False, True):
+        rc, _ = prepared_caster(sc, prm, cams, gpu_device, TuneConfig(replay_words=words, fwd_jump=jump))
+        out = torch.empty((rv.shape[0], 3), device=gpu_device)
+        loss = torch.zeros(1, device=gpu_device)
+        rc.train_batch(rv, rp, target, 0, out, loss)
+        st = rc.stats()
+        res[jump] = (out.cpu().numpy(), float(loss.item()),
+                     {k: v.cpu().numpy() for k, v in rc.finish_grads().items() if v is not None}, st)
+    a, b = res[False], res[True]
+    assert np.array_equal(a[0], b[0]) and abs(a[1] - b[1]) <= 1e-6 * abs(a[1])
+    assert a[3]["slab_hits"] == b[3]["slab_hits"] and a[3]["pairs"] == b[3]["pairs"]
+    assert a[3]["dropped"] == 0 and b[3]["dropped"] == 0
+    for k, v in a[2].items():
+        assert max_rel(b[2][k], v) <= 1e-5, k
+
+
 def test_autotune_picks_a_valid_config(blobs):
     sc, prm, cams, pixels, rc, t, rv, rp, offs = blobs
     """The coordinate search times its starting point and then each axis's other values (one new
